@@ -1,0 +1,217 @@
+"""Python binding of libhos_gpu.so (include/hos_gpu.h) for tests and the benchmark.
+
+The product is the C-ABI library built from ``hos_amd/csrc`` (HIP, gfx950) and
+the C host code in ``hos_amd/host``; this module only loads the library with
+ctypes.  There is no CPU fallback: if the library is missing, or no GPU is
+present, the calls fail loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhos_gpu.so")
+_lib = None
+
+HOS_FP64 = 0
+HOS_FP32 = 1
+
+
+class HosError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("Nx", ctypes.c_int), ("Ny", ctypes.c_int), ("nlevs", ctypes.c_int),
+        ("Lx", ctypes.c_double), ("Ly", ctypes.c_double), ("g", ctypes.c_double),
+        ("depth", ctypes.c_double), ("rampflg", ctypes.c_int), ("Tramp", ctypes.c_double),
+        ("scheme", ctypes.c_int), ("precision", ctypes.c_int), ("kx_quirk", ctypes.c_int),
+        ("mean_level", ctypes.c_int), ("compensated", ctypes.c_int),
+    ]
+
+
+# every symbol include/hos_gpu.h declares
+EXPORTS = [
+    "hos_params_default", "hos_create", "hos_destroy", "hos_footprint", "hos_set_state", "hos_get_state",
+    "hos_set_time", "hos_get_time", "hos_step", "hos_step_dt", "hos_get_energy", "hos_rhs", "hos_zvel",
+    "hos_fft_2d", "hos_ifft_2d", "hos_sync", "hos_last_step_ms", "hos_use_graph", "hos_set_tile",
+    "hos_last_error", "hos_device_name", "hos_num_members", "hos_bytes_per_point_step",
+]
+
+
+def build() -> str:
+    """Compile libhos_gpu.so for gfx950 (hos_amd/Makefile)."""
+    subprocess.run(["make", "-s", "-C", _HERE, "-B", "libhos_gpu.so"], check=True)
+    return LIB_PATH
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HosError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int
+    P = ctypes.POINTER(Params)
+    lib.hos_params_default.argtypes = [P]
+    lib.hos_params_default.restype = None
+    lib.hos_create.argtypes = [P, ip, ctypes.POINTER(vp)]
+    lib.hos_destroy.argtypes = [vp]
+    lib.hos_footprint.argtypes = [P, ip]
+    lib.hos_footprint.restype = ctypes.c_size_t
+    for n in ("hos_set_state", "hos_get_state"):
+        getattr(lib, n).argtypes = [vp, ip, vp]
+    lib.hos_set_time.argtypes = [vp, ip, ctypes.c_double]
+    lib.hos_get_time.argtypes = [vp, ip, dp]
+    lib.hos_step.argtypes = [vp, ip, ctypes.c_double]
+    lib.hos_step_dt.argtypes = [vp, ip, vp]
+    lib.hos_get_energy.argtypes = [vp, ip, dp, dp]
+    lib.hos_rhs.argtypes = [vp, vp, vp, ctypes.c_double]
+    lib.hos_zvel.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.hos_fft_2d.argtypes = [vp, vp, vp]
+    lib.hos_ifft_2d.argtypes = [vp, vp, vp]
+    lib.hos_sync.argtypes = [vp]
+    lib.hos_last_step_ms.argtypes = [vp, dp]
+    lib.hos_use_graph.argtypes = [vp, ip]
+    lib.hos_set_tile.argtypes = [vp, ip]
+    lib.hos_last_error.restype = ctypes.c_char_p
+    lib.hos_device_name.argtypes = [ctypes.c_char_p, ip]
+    lib.hos_num_members.argtypes = [vp]
+    lib.hos_bytes_per_point_step.argtypes = [P]
+    lib.hos_bytes_per_point_step.restype = ctypes.c_double
+    _lib = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise HosError(f"hos error {rc}: {load().hos_last_error().decode()}")
+
+
+def _ptr(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def params(**kw) -> Params:
+    p = Params()
+    load().hos_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def device_name() -> str:
+    b = ctypes.create_string_buffer(256)
+    _check(load().hos_device_name(b, 256))
+    return b.value.decode()
+
+
+def footprint(p: Params, nmembers: int = 1) -> int:
+    return int(load().hos_footprint(ctypes.byref(p), nmembers))
+
+
+def bytes_per_point_step(p: Params) -> float:
+    return float(load().hos_bytes_per_point_step(ctypes.byref(p)))
+
+
+class Solver:
+    """A context of ``nmembers`` instances of one shape (hos_create)."""
+
+    def __init__(self, p: Params, nmembers: int = 1):
+        self.lib = load()
+        self.p = p
+        self.nm = nmembers
+        self.Nx, self.Ny, self.Nk = p.Nx, p.Ny, p.Ny // 2 + 1
+        h = ctypes.c_void_p()
+        _check(self.lib.hos_create(ctypes.byref(p), nmembers, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.hos_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def spec(self, n=1):
+        shape = (self.Nk, self.Nx) if n == 1 else (n, self.Nk, self.Nx)
+        return np.zeros(shape, dtype=np.complex128)
+
+    def set_state(self, u: np.ndarray, member: int = 0):
+        u = np.ascontiguousarray(u, dtype=np.complex128)
+        assert u.shape == (2, self.Nk, self.Nx)
+        _check(self.lib.hos_set_state(self.h, member, _ptr(u)))
+
+    def get_state(self, member: int = 0) -> np.ndarray:
+        u = self.spec(2)
+        _check(self.lib.hos_get_state(self.h, member, _ptr(u)))
+        return u
+
+    def set_time(self, t: float, member: int = 0):
+        _check(self.lib.hos_set_time(self.h, member, t))
+
+    def get_time(self, member: int = 0) -> float:
+        t = ctypes.c_double()
+        _check(self.lib.hos_get_time(self.h, member, ctypes.byref(t)))
+        return t.value
+
+    def step(self, nsteps: int, dt):
+        if np.isscalar(dt):
+            _check(self.lib.hos_step(self.h, nsteps, float(dt)))
+        else:
+            d = np.ascontiguousarray(dt, dtype=np.float64)
+            assert d.shape == (self.nm,)
+            _check(self.lib.hos_step_dt(self.h, nsteps, _ptr(d)))
+
+    def energy(self, member: int = 0):
+        H, v = ctypes.c_double(), ctypes.c_double()
+        _check(self.lib.hos_get_energy(self.h, member, ctypes.byref(H), ctypes.byref(v)))
+        return H.value, v.value
+
+    def rhs(self, u: np.ndarray, t: float = 0.0) -> np.ndarray:
+        u = np.ascontiguousarray(u, dtype=np.complex128)
+        out = self.spec(2)
+        _check(self.lib.hos_rhs(self.h, _ptr(u), _ptr(out), t))
+        return out
+
+    def zvel(self, u: np.ndarray):
+        u = np.ascontiguousarray(u, dtype=np.complex128)
+        outs = [self.spec() for _ in range(4)]
+        _check(self.lib.hos_zvel(self.h, _ptr(u), *[_ptr(o) for o in outs]))
+        return tuple(outs)
+
+    def fft_2d(self, u: np.ndarray) -> np.ndarray:
+        up = np.zeros((self.Nx, self.Ny + 2))
+        up[:, : self.Ny] = u
+        hu = self.spec()
+        _check(self.lib.hos_fft_2d(self.h, _ptr(up), _ptr(hu)))
+        return hu
+
+    def ifft_2d(self, hu: np.ndarray) -> np.ndarray:
+        hu = np.ascontiguousarray(hu, dtype=np.complex128)
+        up = np.zeros((self.Nx, self.Ny + 2))
+        _check(self.lib.hos_ifft_2d(self.h, _ptr(hu), _ptr(up)))
+        return up[:, : self.Ny].copy()
+
+    def last_step_ms(self) -> float:
+        ms = ctypes.c_double()
+        _check(self.lib.hos_last_step_ms(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    def use_graph(self, on: bool = True):
+        _check(self.lib.hos_use_graph(self.h, int(on)))
+
+    def set_tile(self, tx: int):
+        _check(self.lib.hos_set_tile(self.h, tx))
+
+    def sync(self):
+        _check(self.lib.hos_sync(self.h))

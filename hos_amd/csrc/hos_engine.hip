@@ -1,0 +1,891 @@
+// hos_engine.hip -- host orchestration and the native C ABI (include/hos_gpu.h).
+//
+// One context = one grid shape, nm members, one HIP stream.  A step runs
+// nstages RHS evaluations; each RHS is the level schedule
+//
+//   level 0   k_xinv  (eta, eta_x, eta_y, phi_x, phi_y, d^j phi/dz^j j=1..M)
+//             k_yc2r  -> physical fields
+//   level l   k_yr2c  (prologue phi^(l+1))             l = 1..M-1
+//             k_xfwdinv (d^j/dz^j phi^(l+1), j=1..M-l)
+//             k_yc2r  -> physical fields
+//   final     k_yr2c_pair (prologue: nonlinear eta_t, phi_t)
+//             k_xfwd_rhs  (+ W^(1), -g eta, ramp, dealias, RK stage update)
+//
+// which is the reference's rhs_hos + Zvel (hosm_2d_par.c:138-445) with each
+// Mult's r2c/c2r round trip removed: every product is formed on the real grid
+// and only fields that need a spectral operator (d^j/dz^j) are transformed.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hos_gpu.h"
+#include "hos_kernels.hpp"
+
+using namespace hos;
+
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...)
+{
+    char b[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(b, sizeof b, fmt, ap);
+    va_end(ap);
+    g_err = b;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) return fail(HOS_E_DEVICE, "%s: %s", #expr, hipGetErrorString(e_));   \
+    } while (0)
+
+static int ilog2(int n)
+{
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return (1 << l) == n ? l : -1;
+}
+
+constexpr int LOGN_MIN = 4, LOGN_MAX = 11;
+
+struct hos_ctx {
+    hos_params p{};
+    int nm = 0, M = 0, L = 0, Nk = 0, lognx = 0, logny = 0, mx = 0, my = 0;
+    int nI = 0, nP = 0, tx = 0;
+    bool f32 = false;
+    long fs = 0;    // complex elements per spectral field  (Nk*Nx)
+    long fsP = 0;   // real elements per physical field     (Nx*Ny)
+    hipStream_t stream = nullptr;
+    // device memory
+    cx<double> *U = nullptr, *A = nullptr, *Y = nullptr, *K = nullptr;
+    void *I = nullptr, *P = nullptr;
+    void *twx = nullptr, *twy = nullptr;
+    double *kx = nullptr, *t = nullptr, *dt = nullptr, *H = nullptr, *vol = nullptr, *hpart = nullptr,
+           *mean0 = nullptr;
+    double *hostbuf = nullptr;  // device staging [Nx][Ny] double
+    int nK = 0;                 // number of FP64 [2][Nk][Nx] slots in K (per member for RK)
+    HosIdx ix{};
+    // Butcher table, time_schemes_par.c:127-314
+    int S = 0;
+    double a[7] = {}, b[7][7] = {}, c[7] = {};
+    bool subdiag = true;
+    // timing / graph
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    int use_graph = 0;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::vector<double> host_t;  // host mirror of t
+};
+
+extern "C" void hos_params_default(hos_params *p)
+{
+    std::memset(p, 0, sizeof *p);
+    p->Nx = p->Ny = 256;
+    p->nlevs = 2;
+    p->Lx = p->Ly = 2 * M_PI;
+    p->g = 9.81;
+    p->depth = 0.0;
+    p->rampflg = 0;
+    p->Tramp = 1.0;
+    p->scheme = 2;
+    p->precision = HOS_FP64;
+    p->kx_quirk = 1;
+    p->mean_level = 0;
+    p->compensated = 1;
+}
+
+extern "C" const char *hos_last_error(void) { return g_err.c_str(); }
+
+static int setup_scheme(hos_ctx *c, int flg)
+{
+    std::memset(c->a, 0, sizeof c->a);
+    std::memset(c->b, 0, sizeof c->b);
+    std::memset(c->c, 0, sizeof c->c);
+    switch (flg) {
+    case 1:  // Dormand-Prince, 4th-order weights (time_schemes_par.c:181-238)
+        c->S = 7;
+        c->a[1] = 1.0 / 5; c->a[2] = 3.0 / 10; c->a[3] = 4.0 / 5; c->a[4] = 8.0 / 9; c->a[5] = 1; c->a[6] = 1;
+        c->b[1][0] = 1.0 / 5;
+        c->b[2][0] = 3.0 / 40; c->b[2][1] = 9.0 / 40;
+        c->b[3][0] = 44.0 / 45; c->b[3][1] = -56.0 / 15; c->b[3][2] = 32.0 / 9;
+        c->b[4][0] = 19372.0 / 6561; c->b[4][1] = -25360.0 / 2187; c->b[4][2] = 64448.0 / 6561;
+        c->b[4][3] = -212.0 / 729;
+        c->b[5][0] = 9017.0 / 3168; c->b[5][1] = -355.0 / 33; c->b[5][2] = 46732.0 / 5247;
+        c->b[5][3] = 49.0 / 176; c->b[5][4] = -5103.0 / 18656;
+        c->b[6][0] = 35.0 / 384; c->b[6][2] = 500.0 / 1113; c->b[6][3] = 125.0 / 192;
+        c->b[6][4] = -2187.0 / 6784; c->b[6][5] = 11.0 / 84;
+        c->c[0] = 1951.0 / 21600; c->c[2] = 22642.0 / 50085; c->c[3] = 451.0 / 720;
+        c->c[4] = -12231.0 / 42400; c->c[5] = 649.0 / 6300; c->c[6] = 1.0 / 60;
+        break;
+    case 2:  // RK4 (:241-264)
+        c->S = 4;
+        c->a[1] = 0.5; c->a[2] = 0.5; c->a[3] = 1.0;
+        c->b[1][0] = 0.5; c->b[2][1] = 0.5; c->b[3][2] = 1.0;
+        c->c[0] = 1.0 / 6; c->c[1] = 1.0 / 3; c->c[2] = 1.0 / 3; c->c[3] = 1.0 / 6;
+        break;
+    case 3: c->S = 1; c->c[0] = 1.0; break;                                              // Euler
+    case 4: c->S = 2; c->a[1] = 0.5; c->b[1][0] = 0.5; c->c[0] = 0; c->c[1] = 1; break; // midpoint
+    case 5: c->S = 2; c->a[1] = 1; c->b[1][0] = 1; c->c[0] = 0.5; c->c[1] = 0.5; break; // Heun
+    default: return fail(HOS_E_INVALID, "unknown time scheme %d", flg);
+    }
+    c->subdiag = true;
+    for (int s = 0; s < c->S; ++s)
+        for (int q = 0; q < s - 1; ++q)
+            if (c->b[s][q] != 0.0) c->subdiag = false;
+    return HOS_OK;
+}
+
+static int default_tx(bool f32, int logny)
+{
+    // keep the y-pass LDS tile at <= 64 KiB: TX/2 rows of Ny complex values
+    const int bytes = f32 ? 8 : 16;
+    int tx = 16;
+    while (tx > 4 && (tx / 2) * (1 << logny) * bytes > 65536) tx /= 2;
+    if (!f32 && tx > 8) tx = 8;
+    return tx;
+}
+
+static bool tx_valid(bool f32, int tx) { return f32 ? (tx == 8 || tx == 16) : (tx == 4 || tx == 8); }
+
+static int validate(const hos_params *p, int nm)
+{
+    if (!p) return fail(HOS_E_INVALID, "null params");
+    const int lx = ilog2(p->Nx), ly = ilog2(p->Ny);
+    if (lx < LOGN_MIN || lx > LOGN_MAX || ly < LOGN_MIN || ly > LOGN_MAX)
+        return fail(HOS_E_INVALID, "grid %dx%d: Nx and Ny must be powers of two in [%d, %d]", p->Nx, p->Ny,
+                    1 << LOGN_MIN, 1 << LOGN_MAX);
+    if (p->nlevs < 0 || p->nlevs + 1 > MAXM) return fail(HOS_E_INVALID, "nlevs %d out of range [0, %d]", p->nlevs, MAXM - 1);
+    if (!(p->Lx > 0) || !(p->Ly > 0)) return fail(HOS_E_INVALID, "domain lengths must be positive");
+    if (p->precision != HOS_FP64 && p->precision != HOS_FP32) return fail(HOS_E_INVALID, "bad precision");
+    if (p->rampflg == 1 && !(p->Tramp > 0)) return fail(HOS_E_INVALID, "Tramp must be positive");
+    if (nm < 1) return fail(HOS_E_INVALID, "nmembers must be >= 1");
+    return HOS_OK;
+}
+
+static void layout(hos_ctx *c)
+{
+    const hos_params &p = c->p;
+    c->L = p.nlevs;
+    c->M = p.nlevs + 1;
+    c->Nk = p.Ny / 2 + 1;
+    c->lognx = ilog2(p.Nx);
+    c->logny = ilog2(p.Ny);
+    c->mx = (int)std::floor(0.5 * p.Nx / (1 + 0.5 * p.nlevs));  // operators_par.c:290-291
+    c->my = (int)std::floor(0.5 * p.Ny / (1 + 0.5 * p.nlevs));
+    c->f32 = p.precision == HOS_FP32;
+    c->fs = (long)c->Nk * p.Nx;
+    c->fsP = (long)p.Nx * p.Ny;
+    c->nI = 5 + c->M;
+    if (c->nI < 4) c->nI = 4;
+    HosIdx &ix = c->ix;
+    std::memset(&ix, 0, sizeof ix);
+    ix.M = c->M;
+    ix.eta = 0; ix.etax = 1; ix.etay = 2; ix.phix = 3; ix.phiy = 4;
+    int f = 5;
+    for (int a = 1; a <= c->M; ++a)
+        for (int j = 1; j <= c->M - a + 1; ++j) ix.d[a][j] = f++;
+    c->nP = f;
+    double fact = 1.0;
+    ix.invfact[0] = 1.0;
+    for (int k = 1; k <= MAXM; ++k) {
+        fact *= k;
+        ix.invfact[k] = 1.0 / fact;
+    }
+}
+
+static size_t footprint(const hos_ctx &c)
+{
+    const size_t rb = c.f32 ? 4 : 8;
+    size_t b = 0;
+    b += 3 * (size_t)c.nm * 2 * c.fs * 16;         // U, A, Y
+    b += (size_t)c.nK * c.nm * 2 * c.fs * 16;      // K
+    b += (size_t)c.nm * c.nI * c.fs * 2 * rb;      // I
+    b += (size_t)c.nm * c.nP * c.fsP * rb;         // P
+    b += (size_t)c.fsP * 8;                        // staging
+    return b;
+}
+
+extern "C" size_t hos_footprint(const hos_params *p, int nmembers)
+{
+    if (validate(p, nmembers) != HOS_OK) return 0;
+    hos_ctx c;
+    c.p = *p;
+    c.nm = nmembers;
+    layout(&c);
+    setup_scheme(&c, p->scheme);
+    c.nK = c.subdiag ? 0 : c.S;
+    size_t b = footprint(c);
+    b += (size_t)2 * 2 * c.fs * 16;  // K slots for hos_rhs / hos_zvel
+    return b;
+}
+
+template <class T>
+static int upload_twiddles(int n, void **dst)
+{
+    std::vector<cx<T>> h(n);
+    const long double PI = 3.14159265358979323846264338327950288L;
+    for (int k = 0; k < n; ++k) {
+        long double ang = -2.0L * PI * (long double)k / (long double)n;
+        h[k] = {T(cosl(ang)), T(sinl(ang))};
+    }
+    HIP_TRY(hipMalloc(dst, sizeof(cx<T>) * n));
+    HIP_TRY(hipMemcpy(*dst, h.data(), sizeof(cx<T>) * n, hipMemcpyHostToDevice));
+    return HOS_OK;
+}
+
+static void release(hos_ctx *c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    void *ptrs[] = {c->U, c->A, c->Y, c->K, c->I, c->P, c->twx, c->twy, c->kx, c->t, c->dt, c->H, c->vol,
+                    c->hpart, c->mean0, c->hostbuf};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+static int xrows(int lognx) { return lognx >= 10 ? 1 : (1 << (10 - lognx)); }
+
+extern "C" int hos_create(const hos_params *p, int nmembers, hos_ctx **out)
+{
+    if (!out) return fail(HOS_E_INVALID, "null output pointer");
+    *out = nullptr;
+    int rc = validate(p, nmembers);
+    if (rc) return rc;
+    hos_ctx *c = new hos_ctx();
+    c->p = *p;
+    c->nm = nmembers;
+    layout(c);
+    if ((rc = setup_scheme(c, p->scheme))) {
+        delete c;
+        return rc;
+    }
+    c->nK = c->subdiag ? 2 : (c->S > 2 ? c->S : 2);
+    c->tx = default_tx(c->f32, c->logny);
+    const int nm = nmembers;
+    auto bail = [&](int code) {
+        release(c);
+        delete c;
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(HOS_E_DEVICE, "hipStreamCreate failed"));
+    (void)hipEventCreate(&c->ev0);
+    (void)hipEventCreate(&c->ev1);
+    const size_t rb = c->f32 ? 4 : 8;
+    const size_t st = (size_t)nm * 2 * c->fs * sizeof(cx<double>);
+    const size_t kslots = (size_t)(c->subdiag ? 2 : c->S * nm);
+    const size_t kbytes = (kslots < 2 ? 2 : kslots) * 2 * c->fs * sizeof(cx<double>);
+    const int nparts = (c->Nk + xrows(c->lognx) - 1) / xrows(c->lognx);
+    struct {
+        void **p;
+        size_t n;
+    } allocs[] = {{(void **)&c->U, st},
+                  {(void **)&c->A, st},
+                  {(void **)&c->Y, st},
+                  {(void **)&c->K, kbytes},
+                  {&c->I, (size_t)nm * c->nI * c->fs * 2 * rb},
+                  {&c->P, (size_t)nm * c->nP * c->fsP * rb},
+                  {(void **)&c->kx, sizeof(double) * p->Nx},
+                  {(void **)&c->t, sizeof(double) * nm},
+                  {(void **)&c->dt, sizeof(double) * nm},
+                  {(void **)&c->H, sizeof(double) * nm},
+                  {(void **)&c->vol, sizeof(double) * nm},
+                  {(void **)&c->mean0, sizeof(double) * nm},
+                  {(void **)&c->hpart, sizeof(double) * nm * nparts},
+                  {(void **)&c->hostbuf, sizeof(double) * c->fsP}};
+    for (auto &al : allocs) {
+        if (hipMalloc(al.p, al.n) != hipSuccess)
+            return bail(fail(HOS_E_NOMEM, "hipMalloc of %zu bytes failed", al.n));
+        (void)hipMemset(*al.p, 0, al.n);
+    }
+    // kx table, operators_par.c:55-63 (reference) or signed
+    std::vector<double> kx(p->Nx);
+    const double Kx0 = 2 * M_PI / p->Lx;
+    for (int i = 0; i < p->Nx; ++i) {
+        if (p->kx_quirk)
+            kx[i] = (i <= p->Nx / 2 + 1) ? i * Kx0 : -(p->Nx - i) * Kx0;
+        else
+            kx[i] = (i <= p->Nx / 2) ? i * Kx0 : -(p->Nx - i) * Kx0;
+    }
+    if (hipMemcpy(c->kx, kx.data(), sizeof(double) * p->Nx, hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(HOS_E_DEVICE, "kx upload failed"));
+    rc = c->f32 ? upload_twiddles<float>(p->Nx, &c->twx) : upload_twiddles<double>(p->Nx, &c->twx);
+    if (!rc) rc = c->f32 ? upload_twiddles<float>(p->Ny, &c->twy) : upload_twiddles<double>(p->Ny, &c->twy);
+    if (rc) return bail(rc);
+    c->host_t.assign(nm, 0.0);
+    if (hipDeviceSynchronize() != hipSuccess) return bail(fail(HOS_E_DEVICE, "device sync failed"));
+    *out = c;
+    return HOS_OK;
+}
+
+extern "C" int hos_destroy(hos_ctx *c)
+{
+    if (!c) return HOS_OK;
+    (void)hipStreamSynchronize(c->stream);
+    release(c);
+    delete c;
+    return HOS_OK;
+}
+
+extern "C" int hos_num_members(hos_ctx *c) { return c ? c->nm : 0; }
+
+// ---------------------------------------------------------------------------
+// kernel dispatch on (T, log2 N, tile)
+static KTab ktab(const hos_ctx *c)
+{
+    KTab k;
+    k.kx = c->kx;
+    k.ky0 = 2 * M_PI / c->p.Ly;
+    k.depth = c->p.depth;
+    k.nk = c->Nk;
+    k.mx = c->mx;
+    k.my = c->my;
+    return k;
+}
+
+#define LOGN_SWITCH(v, ...)                                                                       \
+    switch (v) {                                                                                   \
+    case 4: { constexpr int LG = 4; __VA_ARGS__; } break;                                                 \
+    case 5: { constexpr int LG = 5; __VA_ARGS__; } break;                                                 \
+    case 6: { constexpr int LG = 6; __VA_ARGS__; } break;                                                 \
+    case 7: { constexpr int LG = 7; __VA_ARGS__; } break;                                                 \
+    case 8: { constexpr int LG = 8; __VA_ARGS__; } break;                                                 \
+    case 9: { constexpr int LG = 9; __VA_ARGS__; } break;                                                 \
+    case 10: { constexpr int LG = 10; __VA_ARGS__; } break;                                               \
+    case 11: { constexpr int LG = 11; __VA_ARGS__; } break;                                               \
+    default: return fail(HOS_E_INVALID, "unsupported size 2^%d", v);                                \
+    }
+
+#define TX_SWITCH(T, tx, ...)                                                                     \
+    if constexpr (sizeof(T) == 4) {                                                                \
+        if (tx == 16) { constexpr int TXV = 16; __VA_ARGS__; }                                            \
+        else { constexpr int TXV = 8; __VA_ARGS__; }                                                      \
+    } else {                                                                                       \
+        if (tx == 8) { constexpr int TXV = 8; __VA_ARGS__; }                                              \
+        else { constexpr int TXV = 4; __VA_ARGS__; }                                                      \
+    }
+
+template <class T>
+struct Eng {
+    hos_ctx *c;
+    int nm;  // members in this launch (from member 0)
+    cx<T> *I() const { return (cx<T> *)c->I; }
+    T *P() const { return (T *)c->P; }
+    long sI() const { return (long)c->nI * c->fs; }
+    long sP() const { return (long)c->nP * c->fsP; }
+    long sY() const { return 2 * c->fs; }
+    const cx<T> *twx() const { return (const cx<T> *)c->twx; }
+    const cx<T> *twy() const { return (const cx<T> *)c->twy; }
+    Phys<T> phys() const { return Phys<T>{P(), sP(), c->fsP, c->p.Ny}; }
+    bool comp() const { return c->f32 && c->p.compensated; }
+
+    int xinv(const cx<double> *Y, const XinvOps &ops, int nops)
+    {
+        LOGN_SWITCH(c->lognx, {
+            constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
+            dim3 grid((c->Nk + ROWS - 1) / ROWS, nops, nm);
+            hipLaunchKernelGGL((k_xinv<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, Y, sY(), I(), sI(), c->fs, ops,
+                               ktab(c), twx());
+        });
+        return HOS_OK;
+    }
+    int yc2r(const FieldMap &fm, int nf)
+    {
+        const int tx = c->tx;
+        LOGN_SWITCH(c->logny, {
+            TX_SWITCH(T, tx, {
+                dim3 grid(c->p.Nx / TXV, nf, nm);
+                hipLaunchKernelGGL((k_yc2r<T, LG, TXV>), grid, dim3(NT), 0, c->stream, I(), sI(), c->fs, P(), sP(),
+                                   c->fsP, fm, c->p.Nx, twy());
+            })
+        });
+        return HOS_OK;
+    }
+    template <class PRO>
+    int yr2c(const PRO &pro, int dst)
+    {
+        const int tx = c->tx;
+        LOGN_SWITCH(c->logny, {
+            TX_SWITCH(T, tx, {
+                dim3 grid(c->p.Nx / TXV, 1, nm);
+                hipLaunchKernelGGL((k_yr2c<T, LG, TXV, PRO>), grid, dim3(NT), 0, c->stream, pro, I(), sI(), c->fs,
+                                   dst, c->p.Nx, twy());
+            })
+        });
+        return HOS_OK;
+    }
+    template <class PRO>
+    int yr2c_pair(const PRO &pro, int da, int db)
+    {
+        const int tx = c->tx;
+        LOGN_SWITCH(c->logny, {
+            TX_SWITCH(T, tx, {
+                constexpr int TXP = TXV / 2;
+                dim3 grid(c->p.Nx / TXP, 1, nm);
+                hipLaunchKernelGGL((k_yr2c_pair<T, LG, TXP, PRO>), grid, dim3(NT), 0, c->stream, pro, I(), sI(),
+                                   c->fs, da, db, c->p.Nx, twy());
+            })
+        });
+        return HOS_OK;
+    }
+    int xfwdinv(int src, int dst0, int ndst)
+    {
+        const double scale = 1.0 / ((double)c->p.Nx * c->p.Ny);
+        LOGN_SWITCH(c->lognx, {
+            constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
+            dim3 grid((c->Nk + ROWS - 1) / ROWS, 1, nm);
+            hipLaunchKernelGGL((k_xfwdinv<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, I(), sI(), c->fs, src, dst0,
+                               ndst, scale, ktab(c), twx());
+        });
+        return HOS_OK;
+    }
+    int xfwd_rhs(int ia, int ib, const cx<double> *Y, const RkArgs &rk)
+    {
+        const double scale = 1.0 / ((double)c->p.Nx * c->p.Ny);
+        LOGN_SWITCH(c->lognx, {
+            constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
+            dim3 grid((c->Nk + ROWS - 1) / ROWS, 1, nm);
+            hipLaunchKernelGGL((k_xfwd_rhs<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, I(), sI(), c->fs, ia, ib, Y,
+                               sY(), scale, ktab(c), rk, twx());
+        });
+        return HOS_OK;
+    }
+
+    // physical fields of the W expansion from the stage input Y
+    int w_levels(const cx<double> *Y)
+    {
+        const HosIdx &ix = c->ix;
+        const int M = c->M;
+        XinvOps ops{};
+        FieldMap fm{};
+        int n = 0;
+        auto add = [&](int src, int op, int pw, int pdst) {
+            ops.o[n] = {src, op, pw, n};
+            fm.src[n] = n;
+            fm.dst[n] = pdst;
+            ++n;
+        };
+        add(0, OP_ID, 0, ix.eta);
+        add(0, OP_DX, 0, ix.etax);
+        add(0, OP_DY, 0, ix.etay);
+        add(1, OP_DX, 0, ix.phix);
+        add(1, OP_DY, 0, ix.phiy);
+        for (int j = 1; j <= M; ++j) add(1, OP_DZ, j, ix.d[1][j]);
+        int rc = xinv(Y, ops, n);
+        if (!rc) rc = yc2r(fm, n);
+        for (int lev = 1; lev <= M - 1 && !rc; ++lev) {
+            ProPhi<T> pro{phys(), ix, lev, comp()};
+            rc = yr2c(pro, 0);
+            const int nd = M - lev;
+            if (!rc) rc = xfwdinv(0, 1, nd);
+            FieldMap f2{};
+            for (int j = 1; j <= nd; ++j) {
+                f2.src[j - 1] = j;
+                f2.dst[j - 1] = ix.d[lev + 1][j];
+            }
+            if (!rc) rc = yc2r(f2, nd);
+        }
+        return rc;
+    }
+
+    int rhs(const cx<double> *Y, const RkArgs &rk)
+    {
+        int rc = w_levels(Y);
+        if (rc) return rc;
+        ProRhs<T> pro{phys(), c->ix, comp()};
+        rc = yr2c_pair(pro, 0, 1);
+        if (!rc) rc = xfwd_rhs(0, 1, Y, rk);
+        return rc;
+    }
+};
+
+static RkArgs rk_base(hos_ctx *c)
+{
+    RkArgs rk{};
+    rk.g = c->p.g;
+    rk.rampflg = c->p.rampflg;
+    rk.tramp = c->p.Tramp;
+    rk.t = c->t;
+    rk.dt = c->dt;
+    rk.U = c->U;
+    rk.A = c->A;
+    rk.Yn = c->Y;
+    rk.K = c->K;
+    rk.sK = 2 * c->fs;
+    rk.hpart = c->hpart;
+    rk.vol = c->vol;
+    rk.mean_level = c->p.mean_level;
+    rk.mean0 = c->mean0;
+    rk.nstages = c->S;
+    return rk;
+}
+
+template <class T>
+static int launch_step(hos_ctx *c)
+{
+    Eng<T> e{c, c->nm};
+    const int nparts = (c->Nk + xrows(c->lognx) - 1) / xrows(c->lognx);
+    int rc = HOS_OK;
+    for (int s = 0; s < c->S && !rc; ++s) {
+        RkArgs rk = rk_base(c);
+        rk.stage = s;
+        rk.c_s = c->c[s];
+        if (c->subdiag) {
+            rk.mode = EPI_RK_SUBDIAG;
+            rk.b_next = (s + 1 < c->S) ? c->b[s + 1][s] : 0.0;
+            rc = e.rhs(c->Y, rk);
+        } else {
+            // generic Butcher: K_s stored, then Y_{s+1} = u + dt sum_p b[s+1][p] K_p
+            rk.mode = EPI_STORE_K;
+            rk.K = c->K + (long)s * c->nm * 2 * c->fs;
+            rc = e.rhs(c->Y, rk);
+            CombArgs ca{};
+            const bool last = (s == c->S - 1);
+            ca.np = s + 1;
+            for (int q = 0; q <= s; ++q) ca.coef[q] = last ? c->c[q] : c->b[s + 1][q];
+            const long n = 2 * c->fs;
+            dim3 grid((unsigned)std::min<long>((n + NT - 1) / NT, 2048), c->nm);
+            if (!rc) {
+                hipLaunchKernelGGL(k_combine, grid, dim3(NT), 0, c->stream, c->U, c->K, (long)c->nm * n, n,
+                                   last ? c->U : c->Y, n, c->dt, ca);
+                if (last)
+                    hipLaunchKernelGGL(k_copy_c64, dim3(2048), dim3(NT), 0, c->stream, c->U, c->Y,
+                                       (long)c->nm * n);
+            }
+        }
+    }
+    if (!rc)
+        hipLaunchKernelGGL(k_end_step, dim3(c->nm), dim3(NT), 0, c->stream, c->hpart, nparts, c->H, c->t, c->dt,
+                           1);
+    return rc;
+}
+
+static int step_once(hos_ctx *c) { return c->f32 ? launch_step<float>(c) : launch_step<double>(c); }
+
+extern "C" int hos_use_graph(hos_ctx *c, int enable)
+{
+    if (!c) return fail(HOS_E_INVALID, "null context");
+    c->use_graph = enable ? 1 : 0;
+    return HOS_OK;
+}
+
+static int build_graph(hos_ctx *c)
+{
+    if (c->gexec) return HOS_OK;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = step_once(c);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(HOS_E_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    c->graph = g;
+    HIP_TRY(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+    return HOS_OK;
+}
+
+extern "C" int hos_step_dt(hos_ctx *c, int nsteps, const double *dt)
+{
+    if (!c || !dt) return fail(HOS_E_INVALID, "null argument");
+    if (nsteps < 0) return fail(HOS_E_INVALID, "nsteps < 0");
+    HIP_TRY(hipMemcpyAsync(c->dt, dt, sizeof(double) * c->nm, hipMemcpyHostToDevice, c->stream));
+    if (c->use_graph) {
+        int rc = build_graph(c);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    for (int n = 0; n < nsteps; ++n) {
+        if (c->use_graph) {
+            HIP_TRY(hipGraphLaunch(c->gexec, c->stream));
+        } else {
+            int rc = step_once(c);
+            if (rc) return rc;
+        }
+    }
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = nsteps > 0 ? ms / nsteps : 0.0;
+    for (int m = 0; m < c->nm; ++m) c->host_t[m] += nsteps * dt[m];
+    return HOS_OK;
+}
+
+extern "C" int hos_step(hos_ctx *c, int nsteps, double dt)
+{
+    if (!c) return fail(HOS_E_INVALID, "null context");
+    std::vector<double> d(c->nm, dt);
+    return hos_step_dt(c, nsteps, d.data());
+}
+
+extern "C" int hos_last_step_ms(hos_ctx *c, double *ms)
+{
+    if (!c || !ms) return fail(HOS_E_INVALID, "null argument");
+    *ms = c->last_ms;
+    return HOS_OK;
+}
+
+extern "C" int hos_sync(hos_ctx *c)
+{
+    if (!c) return fail(HOS_E_INVALID, "null context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return HOS_OK;
+}
+
+extern "C" int hos_set_tile(hos_ctx *c, int tx)
+{
+    if (!c) return fail(HOS_E_INVALID, "null context");
+    if (tx == 0) tx = default_tx(c->f32, c->logny);
+    if (!tx_valid(c->f32, tx)) return fail(HOS_E_INVALID, "tile %d not available for this precision", tx);
+    if (c->p.Nx % tx) return fail(HOS_E_INVALID, "tile %d does not divide Nx", tx);
+    const int bytes = c->f32 ? 8 : 16;
+    if ((tx / 2) * (1 << c->logny) * bytes > 65536) return fail(HOS_E_INVALID, "tile %d exceeds 64 KiB LDS", tx);
+    c->tx = tx;
+    if (c->gexec) {
+        (void)hipGraphExecDestroy(c->gexec);
+        (void)hipGraphDestroy(c->graph);
+        c->gexec = nullptr;
+        c->graph = nullptr;
+    }
+    return HOS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// state in / out
+static int check_member(hos_ctx *c, int m)
+{
+    if (!c) return fail(HOS_E_INVALID, "null context");
+    if (m < 0 || m >= c->nm) return fail(HOS_E_INVALID, "member %d out of range [0, %d)", m, c->nm);
+    return HOS_OK;
+}
+
+extern "C" int hos_set_state(hos_ctx *c, int member, const double *u)
+{
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    if (!u) return fail(HOS_E_INVALID, "null state");
+    const size_t n = 2 * c->fs * sizeof(cx<double>);
+    HIP_TRY(hipMemcpyAsync(c->U + member * 2 * c->fs, u, n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->Y + member * 2 * c->fs, u, n, hipMemcpyHostToDevice, c->stream));
+    const double m0 = u[0];  // Re eta-hat(0,0)
+    HIP_TRY(hipMemcpyAsync(c->mean0 + member, &m0, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return HOS_OK;
+}
+
+extern "C" int hos_get_state(hos_ctx *c, int member, double *u)
+{
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    if (!u) return fail(HOS_E_INVALID, "null state");
+    HIP_TRY(hipMemcpyAsync(u, c->U + member * 2 * c->fs, 2 * c->fs * sizeof(cx<double>), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return HOS_OK;
+}
+
+extern "C" int hos_set_time(hos_ctx *c, int member, double t)
+{
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    c->host_t[member] = t;
+    HIP_TRY(hipMemcpyAsync(c->t + member, &c->host_t[member], sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return HOS_OK;
+}
+
+extern "C" int hos_get_time(hos_ctx *c, int member, double *t)
+{
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    if (!t) return fail(HOS_E_INVALID, "null argument");
+    HIP_TRY(hipMemcpyAsync(t, c->t + member, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return HOS_OK;
+}
+
+extern "C" int hos_get_energy(hos_ctx *c, int member, double *H, double *volume)
+{
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    if (H) HIP_TRY(hipMemcpyAsync(H, c->H + member, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (volume) HIP_TRY(hipMemcpyAsync(volume, c->vol + member, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return HOS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// single-evaluation entry points on member 0 (parity / reference shims).
+// They use member 0's scratch and restore its stage input afterwards.
+static int restore_member0(hos_ctx *c)
+{
+    HIP_TRY(hipMemcpyAsync(c->Y, c->U, 2 * c->fs * sizeof(cx<double>), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return HOS_OK;
+}
+
+template <class T>
+static int rhs_impl(hos_ctx *c, double t)
+{
+    Eng<T> e{c, 1};
+    RkArgs rk = rk_base(c);
+    rk.mode = EPI_STORE_K;
+    rk.stage = 1;  // no Hamiltonian partials
+    rk.hpart = nullptr;
+    rk.K = c->K;
+    HIP_TRY(hipMemcpy(c->t, &t, sizeof(double), hipMemcpyHostToDevice));
+    int rc = e.rhs(c->Y, rk);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(c->t, &c->host_t[0], sizeof(double), hipMemcpyHostToDevice));
+    return rc;
+}
+
+extern "C" int hos_rhs(hos_ctx *c, const double *u, double *hrhs, double t)
+{
+    if (!c || !u || !hrhs) return fail(HOS_E_INVALID, "null argument");
+    const size_t n = 2 * c->fs * sizeof(cx<double>);
+    HIP_TRY(hipMemcpyAsync(c->Y, u, n, hipMemcpyHostToDevice, c->stream));
+    int rc = c->f32 ? rhs_impl<float>(c, t) : rhs_impl<double>(c, t);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(hrhs, c->K, n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipGetLastError());
+    return restore_member0(c);
+}
+
+template <class T>
+static int zvel_impl(hos_ctx *c)
+{
+    Eng<T> e{c, 1};
+    int rc = e.w_levels(c->Y);
+    RkArgs rk = rk_base(c);
+    rk.mode = EPI_STORE_SPEC2;
+    rk.hpart = nullptr;
+    for (int which = 0; which < 2 && !rc; ++which) {
+        ProZvel<T> pro{e.phys(), c->ix, which, e.comp()};
+        rc = e.yr2c_pair(pro, 0, 1);
+        rk.K = c->K + (long)which * 2 * c->fs;
+        if (!rc) rc = e.xfwd_rhs(0, 1, c->Y, rk);
+    }
+    return rc;
+}
+
+extern "C" int hos_zvel(hos_ctx *c, const double *u, double *wM, double *wM2, double *w2M, double *w2M2)
+{
+    if (!c || !u) return fail(HOS_E_INVALID, "null argument");
+    HIP_TRY(hipMemcpyAsync(c->Y, u, 2 * c->fs * sizeof(cx<double>), hipMemcpyHostToDevice, c->stream));
+    int rc = c->f32 ? zvel_impl<float>(c) : zvel_impl<double>(c);
+    if (rc) return rc;
+    const size_t n = c->fs * sizeof(cx<double>);
+    double *outs[4] = {wM, wM2, w2M, w2M2};
+    for (int q = 0; q < 4; ++q)
+        if (outs[q]) HIP_TRY(hipMemcpyAsync(outs[q], c->K + (long)q * c->fs, n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipGetLastError());
+    return restore_member0(c);
+}
+
+template <class T>
+static int fft_impl(hos_ctx *c)
+{
+    Eng<T> e{c, 1};
+    const long n = c->fsP;
+    hipLaunchKernelGGL(k_real_from_f64<T>, dim3(1024), dim3(NT), 0, c->stream, c->hostbuf, e.P(), n);
+    ProRead<T> pro{e.phys(), 0};
+    int rc = e.yr2c(pro, 0);
+    RkArgs rk = rk_base(c);
+    rk.mode = EPI_STORE_SPEC2;
+    rk.hpart = nullptr;
+    rk.K = c->K;
+    if (!rc) rc = e.xfwd_rhs(0, 0, c->Y, rk);
+    return rc;
+}
+
+extern "C" int hos_fft_2d(hos_ctx *c, const double *u_padded, double *hu)
+{
+    if (!c || !u_padded || !hu) return fail(HOS_E_INVALID, "null argument");
+    const int Nx = c->p.Nx, Ny = c->p.Ny;
+    HIP_TRY(hipMemcpy2DAsync(c->hostbuf, sizeof(double) * Ny, u_padded, sizeof(double) * (Ny + 2),
+                             sizeof(double) * Ny, Nx, hipMemcpyHostToDevice, c->stream));
+    int rc = c->f32 ? fft_impl<float>(c) : fft_impl<double>(c);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(hu, c->K, c->fs * sizeof(cx<double>), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return HOS_OK;
+}
+
+template <class T>
+static int ifft_impl(hos_ctx *c)
+{
+    Eng<T> e{c, 1};
+    XinvOps ops{};
+    ops.o[0] = {0, OP_ID, 0, 0};
+    FieldMap fm{};
+    fm.src[0] = 0;
+    fm.dst[0] = 0;
+    int rc = e.xinv(c->Y, ops, 1);
+    if (!rc) rc = e.yc2r(fm, 1);
+    hipLaunchKernelGGL(k_real_to_f64<T>, dim3(1024), dim3(NT), 0, c->stream, e.P(), c->hostbuf, c->fsP);
+    return rc;
+}
+
+extern "C" int hos_ifft_2d(hos_ctx *c, const double *hu, double *u_padded)
+{
+    if (!c || !u_padded || !hu) return fail(HOS_E_INVALID, "null argument");
+    const int Nx = c->p.Nx, Ny = c->p.Ny;
+    HIP_TRY(hipMemcpyAsync(c->Y, hu, c->fs * sizeof(cx<double>), hipMemcpyHostToDevice, c->stream));
+    int rc = c->f32 ? ifft_impl<float>(c) : ifft_impl<double>(c);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy2DAsync(u_padded, sizeof(double) * (Ny + 2), c->hostbuf, sizeof(double) * Ny,
+                             sizeof(double) * Ny, Nx, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int x = 0; x < Nx; ++x) u_padded[(size_t)x * (Ny + 2) + Ny] = u_padded[(size_t)x * (Ny + 2) + Ny + 1] = 0.0;
+    return restore_member0(c);
+}
+
+// ---------------------------------------------------------------------------
+extern "C" int hos_device_name(char *buf, int len)
+{
+    if (!buf || len <= 0) return fail(HOS_E_INVALID, "bad buffer");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    snprintf(buf, len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    return HOS_OK;
+}
+
+extern "C" double hos_bytes_per_point_step(const hos_params *p)
+{
+    // F(M) = M(M+1)/2 + M + 6 two-dimensional real FFTs per RHS, each two
+    // passes (row pass, transposed column pass) that read and write N*p bytes;
+    // plus 64 B of FP64 RK state traffic per point per stage.
+    const int M = p->nlevs + 1;
+    const double F = M * (M + 1) / 2.0 + M + 6;
+    const double pb = p->precision == HOS_FP32 ? 4.0 : 8.0;
+    int S = 4;
+    switch (p->scheme) {
+    case 1: S = 7; break;
+    case 3: S = 1; break;
+    case 4:
+    case 5: S = 2; break;
+    default: S = 4;
+    }
+    return S * (F * 4 * pb + 64.0);
+}

@@ -1,0 +1,148 @@
+/*
+ * hos_gpu.h -- C ABI of the MI355X (gfx950) HOS time-step library.
+ *
+ * Two layers:
+ *
+ *  1. The native API (hos_*): a context holds one or more solver instances
+ *     ("members") of one grid shape, their device-resident state and the
+ *     precomputed FFT tables.  Every entry point returns 0 on success and a
+ *     negative HOS_E* code on failure; hos_last_error() gives the message.
+ *     All array arguments are HOST pointers in the reference's layouts:
+ *       spectral field   double _Complex [Ny/2+1][Nx]   (index Nx*j + i,
+ *                        FFT_TRANSPOSE 1, Csource/2dpar/euler_2d_par.h:22)
+ *       state u          [2][Ny/2+1][Nx]  (eta-hat, then phi-hat at offset
+ *                        Nx*(Ny/2+1), Csource/2dpar/main_2d_par.c:150-158)
+ *       real field       double [Nx][Ny+2] (padded rows,
+ *                        Csource/2dpar/fft_routines_2d_par.c:24-34)
+ *
+ *  2. include/hos_ref.h: the reference's own function names and signatures
+ *     (Setup_TimeScheme, rhs_hos_setup, sol_update_RK, rhs_hos, Zvel, ...),
+ *     implemented on top of layer 1 so that a C driver written against
+ *     Csource/2dpar links against this library instead.
+ *
+ * Thread safety: a context must not be used from two host threads at once.
+ */
+#ifndef HOS_GPU_H
+#define HOS_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define HOS_OK 0
+#define HOS_E_INVALID (-1)   /* bad argument or configuration */
+#define HOS_E_DEVICE (-2)    /* HIP runtime error */
+#define HOS_E_NOMEM (-3)     /* device allocation failed */
+#define HOS_E_STATE (-4)     /* call not valid in the current state */
+
+/* precision of the FFTs, spectral multiplies and products.  The RK state,
+ * the stage combination, the linear terms (-g eta, W^(1)) and the
+ * diagnostics are always FP64. */
+#define HOS_FP64 0
+#define HOS_FP32 1
+
+typedef struct hos_params {
+    int Nx, Ny;        /* grid (powers of two, 16..4096)                      */
+    int nlevs;         /* NLevs of Csource/2dpar/euler_2d_par.h:24; M = nlevs+1 */
+    double Lx, Ly;     /* domain; Kx0 = 2pi/Lx (hdf5_routines_2d_par.c:98-99) */
+    double g;
+    double depth;      /* <= 0: deep water (the reference's live Dz);
+                          > 0: finite depth h, d^n/dz^n -> k^n tanh(kh)^(n odd) */
+    int rampflg;       /* Dommermuth ramp, hosm_2d_par.c:211-229 */
+    double Tramp;
+    int scheme;        /* Setup_TimeScheme flag, time_schemes_par.c:127-314:
+                          1 Dormand-Prince (4th-order weights), 2 RK4,
+                          3 Euler, 4 midpoint, 5 Heun */
+    int precision;     /* HOS_FP64 or HOS_FP32 */
+    int kx_quirk;      /* 1: reference kx table (i <= Nx/2+1 -> +i Kx0,
+                          operators_par.c:55); 0: signed table */
+    int mean_level;    /* 1: hold eta-hat(0,0) at its value when the state was
+                          set (mean-level correction); 0: reference behaviour */
+    int compensated;   /* FP32 only: Kahan-compensated order sums */
+} hos_params;
+
+/* Defaults: deep water, RK4, FP64, reference kx table, no ramp, g = 9.81. */
+void hos_params_default(hos_params *p);
+
+typedef struct hos_ctx hos_ctx;
+
+/* Create a context with nmembers instances of the same shape on the current
+ * HIP device.  Replaces fftw_mpi_local_size_2d_transposed + the FFTW plans
+ * (main_2d_par.c:143,189-190) and rhs_hos_setup / Setup_TimeScheme
+ * allocations (hosm_2d_par.c:51-97, time_schemes_par.c:157-163). */
+int hos_create(const hos_params *p, int nmembers, hos_ctx **out);
+int hos_destroy(hos_ctx *c);
+
+/* Bytes of device memory a context of this shape would allocate. */
+size_t hos_footprint(const hos_params *p, int nmembers);
+
+/* Copy one member's state in / out (host [2][Ny/2+1][Nx] complex double).
+ * hos_set_state applies no dealiasing: callers pass a masked state, as
+ * main_2d_par.c:258-262 does (fft_2d then Dealias). */
+int hos_set_state(hos_ctx *c, int member, const double *u);
+int hos_get_state(hos_ctx *c, int member, double *u);
+
+/* Per-member time t (double), advanced by dt every step, as
+ * time_schemes_par.c:119. */
+int hos_set_time(hos_ctx *c, int member, double t);
+int hos_get_time(hos_ctx *c, int member, double *t);
+
+/* Advance every member by nsteps sol_update_RK steps of size dt
+ * (time_schemes_par.c:28-121); stage time is t_n for every stage (:70).
+ * The stage-0 Hamiltonian of each step is recorded (hos_get_energy). */
+int hos_step(hos_ctx *c, int nsteps, double dt);
+
+/* Same, with a per-member dt array (length nmembers). */
+int hos_step_dt(hos_ctx *c, int nsteps, const double *dt);
+
+/* Stage-0 Hamiltonian of the last step, hosm_2d_par.c:462-523 (the
+ * reference's H = mean(g eta^2 + phi eta_t), no 1/2), and volume = mean eta =
+ * Re eta-hat(0,0), both FP64. */
+int hos_get_energy(hos_ctx *c, int member, double *H, double *volume);
+
+/* One RHS evaluation (rhs_hos, hosm_2d_par.c:138-235) of member 0 on a host
+ * state u at time t; result into host hrhs [2][Ny/2+1][Nx].  Does not change
+ * the context's state. */
+int hos_rhs(hos_ctx *c, const double *u, double *hrhs, double t);
+
+/* W expansion of a host state (Zvel, hosm_2d_par.c:240-445): W^(M), W^(M-2),
+ * (W^2)^(M), (W^2)^(M-2), each host [Ny/2+1][Nx] complex.  Any pointer may be
+ * NULL. */
+int hos_zvel(hos_ctx *c, const double *u, double *wM, double *wM2, double *w2M, double *w2M2);
+
+/* fft_2d / ifft_2d of fft_routines_2d_par.c:21-103 on host arrays:
+ * real [Nx][Ny+2] padded <-> spectral [Ny/2+1][Nx], r2c normalised by
+ * 1/(Nx*Ny), c2r unnormalised.  c2r uses the real part of the ky=0 and
+ * ky=Ny/2 columns after the x inverse (c2r of the Hermitian projection). */
+int hos_fft_2d(hos_ctx *c, const double *u_padded, double *hu);
+int hos_ifft_2d(hos_ctx *c, const double *hu, double *u_padded);
+
+/* Device synchronisation and per-step timing of the last hos_step call
+ * (milliseconds per step from HIP events). */
+int hos_sync(hos_ctx *c);
+int hos_last_step_ms(hos_ctx *c, double *ms);
+
+/* Capture the launches of one step into a hipGraph and replay it for every
+ * later hos_step of the same dt (1) or launch stream-ordered (0). */
+int hos_use_graph(hos_ctx *c, int enable);
+
+/* Launch-shape overrides chosen by the autotuner (0 = built-in default):
+ * tx = x-rows per y-pass workgroup (4, 8 or 16). */
+int hos_set_tile(hos_ctx *c, int tx);
+
+const char *hos_last_error(void);
+
+/* Introspection used by tests and the benchmark. */
+int hos_device_name(char *buf, int len);
+int hos_num_members(hos_ctx *c);
+/* Algorithmic HBM bytes per grid point per step of this configuration's
+ * dominant kernels (FFT passes plus FP64 state traffic). */
+double hos_bytes_per_point_step(const hos_params *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HOS_GPU_H */

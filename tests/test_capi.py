@@ -1,0 +1,41 @@
+"""The C-ABI library loads without a GPU and exports every declared symbol."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(hos_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol(hos):
+    lib = ctypes.CDLL(hos.LIB_PATH)
+    names = declared("hos_gpu.h")
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(names) == sorted(hos.EXPORTS)
+
+
+def test_params_default_and_footprint(hos):
+    p = hos.params(Nx=1024, Ny=1024, nlevs=4, precision=hos.HOS_FP32)
+    assert p.scheme == 2 and p.kx_quirk == 1 and p.g == pytest.approx(9.81)
+    fp = hos.footprint(p, 1)
+    # 20 physical + 10 intermediate FP32 fields, FP64 state/accumulator/stage
+    assert 100e6 < fp < 300e6
+    bad = hos.params(Nx=1000, Ny=1024)
+    assert hos.footprint(bad, 1) == 0
+
+
+def test_bytes_per_point_model(hos):
+    # B(M,p) = S * (F(M)*4p + 64), F(M) = M(M+1)/2 + M + 6
+    p = hos.params(nlevs=4, precision=hos.HOS_FP32)
+    assert hos.bytes_per_point_step(p) == 4 * (26 * 16 + 64)
+    p = hos.params(nlevs=4, precision=hos.HOS_FP64)
+    assert hos.bytes_per_point_step(p) == 4 * (26 * 32 + 64)

@@ -1,0 +1,146 @@
+"""GPU parity: libhos_gpu.so (C ABI) against the FP64 oracle on the same inputs.
+
+Tolerances: FP64 path <= 1e-12 relative L2 for one RHS / transform (both are
+FP64 pseudo-spectral evaluations of the same expressions; they differ only in
+rounding order), FP32 path <= 2e-5 relative L2 for one RHS.
+"""
+import numpy as np
+import pytest
+
+from helpers import random_state, to_spec
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 2e-5
+
+
+def make(hos, Nx, Ny, nlevs, prec, **kw):
+    p = hos.params(Nx=Nx, Ny=Ny, nlevs=nlevs, Lx=kw.pop("Lx", 2 * np.pi), Ly=kw.pop("Ly", 2 * np.pi),
+                   precision=prec, **kw)
+    return hos.Solver(p)
+
+
+@pytest.mark.parametrize("Nx,Ny", [(16, 16), (64, 32), (32, 128), (256, 256), (2048, 512), (512, 2048)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_fft_2d_vs_numpy(hos, Nx, Ny, prec):
+    print(f"fft {Nx}x{Ny} prec={prec}", flush=True)
+    s = make(hos, Nx, Ny, 2, prec)
+    rng = np.random.default_rng(Nx + Ny)
+    u = rng.standard_normal((Nx, Ny))
+    hu = s.fft_2d(u)
+    ref = to_spec(u)
+    tol = 1e-14 if prec == 0 else 3e-7
+    assert rel_l2(hu, ref) < tol
+    back = s.ifft_2d(ref)
+    assert rel_l2(back, u) < tol
+
+
+@pytest.mark.parametrize("nlevs", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("depth", [0.0, 0.7])
+def test_rhs_fp64_vs_oracle(hos, oracle_mod, nlevs, depth):
+    Nx, Ny = 64, 32
+    print(f"rhs fp64 nlevs={nlevs} depth={depth}", flush=True)
+    Lx, Ly = 4 * np.pi, 2 * np.pi
+    o = oracle_mod.Oracle(Nx, Ny, nlevs, Lx, Ly, depth=depth)
+    s = make(hos, Nx, Ny, nlevs, 0, Lx=Lx, Ly=Ly, depth=depth)
+    u = random_state(Nx, Ny, nlevs, amp=0.08, seed=nlevs)
+    ref = o.rhs_hos(u)
+    got = s.rhs(u)
+    assert rel_l2(got, ref) < TOL64, rel_l2(got, ref)
+
+
+@pytest.mark.parametrize("quirk", [0, 1])
+@pytest.mark.parametrize("ramp", [0, 1])
+def test_rhs_quirk_ramp(hos, oracle_mod, quirk, ramp):
+    Nx, Ny, nlevs = 32, 32, 2
+    print(f"rhs quirk={quirk} ramp={ramp}", flush=True)
+    o = oracle_mod.Oracle(Nx, Ny, nlevs, 2 * np.pi, 2 * np.pi, rampflg=ramp, Tramp=3.0, kx_quirk=bool(quirk))
+    s = make(hos, Nx, Ny, nlevs, 0, rampflg=ramp, Tramp=3.0, kx_quirk=quirk)
+    u = random_state(Nx, Ny, nlevs, amp=0.1, seed=3)
+    for t in (0.0, 1.7):
+        assert rel_l2(s.rhs(u, t), o.rhs_hos(u, t)) < TOL64
+
+
+@pytest.mark.parametrize("nlevs", [1, 2, 4])
+def test_rhs_fp32_vs_oracle(hos, oracle_mod, nlevs):
+    Nx, Ny = 128, 64
+    print(f"rhs fp32 nlevs={nlevs}", flush=True)
+    o = oracle_mod.Oracle(Nx, Ny, nlevs, 2 * np.pi, np.pi)
+    s = make(hos, Nx, Ny, nlevs, 1, Ly=np.pi)
+    u = random_state(Nx, Ny, nlevs, amp=0.05, seed=7)
+    assert rel_l2(s.rhs(u), o.rhs_hos(u)) < TOL32
+
+
+@pytest.mark.parametrize("nlevs", [1, 2, 3, 4])
+def test_zvel_vs_oracle(hos, oracle_mod, nlevs):
+    Nx, Ny = 32, 64
+    print(f"zvel nlevs={nlevs}", flush=True)
+    o = oracle_mod.Oracle(Nx, Ny, nlevs, 2 * np.pi, 2 * np.pi)
+    s = make(hos, Nx, Ny, nlevs, 0)
+    u = random_state(Nx, Ny, nlevs, amp=0.1, seed=11)
+    ref = o.Zvel(u)
+    got = s.zvel(u)
+    for r, g in zip(ref, got):
+        if np.linalg.norm(r) == 0:
+            assert np.linalg.norm(g) < 1e-14
+        else:
+            assert rel_l2(g, r) < TOL64
+
+
+@pytest.mark.parametrize("scheme", [1, 2, 3, 4, 5])
+def test_steps_vs_oracle(hos, oracle_mod, scheme):
+    Nx, Ny, nlevs = 32, 32, 2
+    print(f"steps scheme={scheme}", flush=True)
+    o = oracle_mod.Oracle(Nx, Ny, nlevs, 2 * np.pi, 2 * np.pi, scheme=scheme)
+    s = make(hos, Nx, Ny, nlevs, 0, scheme=scheme)
+    u = random_state(Nx, Ny, nlevs, amp=0.05, seed=5)
+    s.set_state(u)
+    dt = 0.05
+    ur = u.copy()
+    t = 0.0
+    Hs = []
+    for n in range(10):
+        t, H = o.step(ur, t, dt)
+        Hs.append(H)
+    s.step(10, dt)
+    got = s.get_state()
+    assert rel_l2(got, ur) < 1e-12
+    Hg, vol = s.energy()
+    assert abs(Hg - Hs[-1]) <= 1e-12 * abs(Hs[-1])
+    assert abs(s.get_time() - t) < 1e-12
+
+
+def test_members_match_alone(hos):
+    """Each member of a batch gives the result it gives alone (bit-identical)."""
+    Nx, Ny, nlevs = 64, 64, 3
+    print("members", flush=True)
+    p = hos.params(Nx=Nx, Ny=Ny, nlevs=nlevs, precision=1)
+    states = [random_state(Nx, Ny, nlevs, amp=0.06, seed=20 + i) for i in range(3)]
+    batch = hos.Solver(p, 3)
+    for i, u in enumerate(states):
+        batch.set_state(u, i)
+    dts = np.array([0.02, 0.03, 0.025])
+    batch.step(5, dts)
+    for i, u in enumerate(states):
+        alone = hos.Solver(p, 1)
+        alone.set_state(u)
+        alone.step(5, dts[i])
+        assert np.array_equal(alone.get_state(), batch.get_state(i))
+
+
+def test_graph_replay_matches_eager(hos):
+    Nx, Ny, nlevs = 64, 64, 4
+    print("graph", flush=True)
+    p = hos.params(Nx=Nx, Ny=Ny, nlevs=nlevs, precision=1)
+    u = random_state(Nx, Ny, nlevs, amp=0.06, seed=9)
+    a = hos.Solver(p)
+    b = hos.Solver(p)
+    a.set_state(u)
+    b.set_state(u)
+    b.use_graph(True)
+    for rep in range(3):
+        a.step(4, 0.02)
+        b.step(4, 0.02)
+        assert np.array_equal(a.get_state(), b.get_state())
