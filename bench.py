@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Benchmark of the HOS time-step path on one MI355X.
+
+Workload (the single-instance 1024^2 configuration): 1024x1024 grid over a
+32x32 peak-wavelength square domain, kp = 1 m^-1, finite depth kp*h = 1.0,
+HOS order M = 5 (nlevs = 4), JONSWAP gamma = 5, kp*Hs/2 = 0.12, cos^2
+spreading, |k| <= 4 kp, seed 1, classical RK4 with dt = Tp/100.
+
+A step is one RK4 step (4 RHS evaluations) of every member.  The FP32 form
+(FFTs, spectral multiplies and products in FP32 with Kahan-compensated order
+sums; FP64 state, stage combination, linear terms and diagnostics) is timed
+first, then the plain FP64 form on the same input.  value = grid-point RK4
+steps/s of the FP32 form, state resident in HBM, timed between two device
+synchronisations.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+
+
+def workload():
+    kp = 1.0
+    L = 32 * 2 * math.pi / kp
+    return dict(Nx=1024, Ny=1024, Lx=L, Ly=L, kp=kp, depth=1.0 / kp, nlevs=4, gamma=5.0, eps=0.12, seed=1,
+                g=9.81)
+
+
+def make_state(hos, w):
+    return hos.jonswap(w["Nx"], w["Ny"], w["Lx"], w["Ly"], kp=w["kp"], gamma=w["gamma"], eps=w["eps"],
+                       depth=w["depth"], g=w["g"], seed=w["seed"], nlevs=w["nlevs"])
+
+
+def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_steps=0):
+    p = hos.params(Nx=w["Nx"], Ny=w["Ny"], nlevs=w["nlevs"], Lx=w["Lx"], Ly=w["Ly"], g=w["g"],
+                   depth=w["depth"], precision=prec, compensated=1)
+    s = hos.Solver(p, members)
+    for m in range(members):
+        s.set_state(u0, m)
+    H0, v0 = None, None
+    s.use_graph(graph)
+    s.step(warmup, dt)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(steps, dt)  # returns after the device has finished the K steps
+    s.sync()
+    wall = time.perf_counter() - t0
+    st = s.step_times()
+    out = {
+        "wall_s": wall,
+        "ms_per_step": 1e3 * wall / steps,
+        "value": members * w["Nx"] * w["Ny"] * steps / wall,
+        "p50_ms": float(np.percentile(st, 50)),
+        "p99_ms": float(np.percentile(st, 99)),
+        "H": s.energy(0)[0],
+        "vol": s.energy(0)[1],
+        "state": s.get_state(0),
+        "bytes_per_pt_step": hos.bytes_per_point_step(p),
+    }
+    if profile_steps:
+        # same workload, stream-ordered launches with a HIP event pair around
+        # every kernel on the solver's stream
+        s.use_graph(False)
+        s.profile(True)
+        s.step(profile_steps, dt)
+        out["kstats"] = s.kernel_stats()
+        out["profile_ms_per_step"] = s.last_step_ms()
+        s.profile(False)
+    s.close()
+    return out
+
+
+def cpu_baseline(w, dt, u0):
+    """The FP64 oracle (C restatement of the reference path) on one RK4 step of
+    the same workload, one host thread."""
+    import oracle
+
+    o = oracle.Oracle(w["Nx"], w["Ny"], w["nlevs"], w["Lx"], w["Ly"], g=w["g"], depth=w["depth"], scheme=2)
+    u = np.ascontiguousarray(u0.copy())
+    nsteps = 1
+    t0 = time.perf_counter()
+    o.run(u, 0.0, dt, nsteps)
+    sec = time.perf_counter() - t0
+    return {
+        "value": w["Nx"] * w["Ny"] * nsteps / sec,
+        "unit": "grid-pt RK4 steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{nsteps} RK4 step of the same 1024^2 M=5 workload ({sec:.1f} s), FP64, reference "
+                  "Mult structure (3 FFTs per product), own radix-2 FFT; the reference FFTW-MPI build is not "
+                  "buildable in this image",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--members", type=int, default=1)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); members of an ensemble "
+              "shard across GPUs with no communication, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import hos_amd as hos
+
+    w = workload()
+    Tp = hos.peak_period(w["g"], w["kp"], w["depth"])
+    dt = Tp / 100.0
+    u0 = make_state(hos, w)
+    print(f"[bench] device: {hos.device_name()}", file=sys.stderr, flush=True)
+
+    f32 = run_form(hos, w, hos.HOS_FP32, args.members, args.steps, args.warmup, dt, u0,
+                   graph=not args.no_graph, profile_steps=args.profile_steps)
+    print(f"[bench] fp32: {f32['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    f64 = None
+    if not args.no_fp64:
+        f64 = run_form(hos, w, hos.HOS_FP64, 1, args.steps, args.warmup, dt, u0, graph=not args.no_graph)
+        print(f"[bench] fp64: {f64['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+
+    # dominant kernel: largest summed duration in the profiled pass
+    ks = f32.get("kstats", {})
+    roof = None
+    if ks:
+        name = max(ks, key=lambda k: ks[k]["ms"])
+        k = ks[name]
+        avg_ms = k["ms"] / max(k["launches"], 1)
+        avg_bytes = k["bytes"] / max(k["launches"], 1)
+        achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+        roof = {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+            "avg_launch_us": round(avg_ms * 1e3, 2), "bytes_per_launch": avg_bytes,
+            "kernel_share": round(k["ms"] / sum(v["ms"] for v in ks.values()), 3),
+            "per_kernel": {n: {"launches": v["launches"],
+                               "avg_us": round(1e3 * v["ms"] / max(v["launches"], 1), 2),
+                               "GBps": round(v["bytes"] / max(v["ms"], 1e-12) / 1e6, 1)}
+                           for n, v in ks.items() if v["launches"]},
+        }
+    cpu = None if args.no_cpu else cpu_baseline(w, dt, u0)
+
+    res = {
+        "metric": "grid-pt RK4 steps/s, M=5 1024^2, FP32 batched vs FP64 plain; p50/p99 ms",
+        "value": round(f32["value"], 1),
+        "unit": "grid-pt RK4 steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(f32["ms_per_step"], 4),
+        "p50_ms": round(f32["p50_ms"], 4),
+        "p99_ms": round(f32["p99_ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic seeded JONSWAP sea state (hos_ic_jonswap)",
+        "config": {
+            "workload": "1024x1024 grid, 32x32 peak wavelengths, kp*h=1, M=5, JONSWAP gamma=5 kp*Hs/2=0.12 "
+                        "seed 1, RK4 dt=Tp/100",
+            "members": args.members, "graph": not args.no_graph,
+            "form": "FP32 FFTs/products with Kahan order sums; FP64 state, RK combination, linear terms",
+            "parallelism": "single GPU",
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if f64:
+        res["fp64_plain"] = {"value": round(f64["value"], 1), "ms_per_step": round(f64["ms_per_step"], 4),
+                             "p50_ms": round(f64["p50_ms"], 4), "p99_ms": round(f64["p99_ms"], 4)}
+        res["speedup_fp32_vs_fp64"] = round(f32["value"] / f64["value"], 3)
+        d = f32["state"][0] - f64["state"][0]
+        res["eta_rel_l2_fp32_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(f64["state"][0]))
+        res["steps_compared"] = args.warmup + args.steps
+    res["alg_bytes_per_pt_step"] = f32["bytes_per_pt_step"]
+    res["alg_GBps"] = round(f32["bytes_per_pt_step"] * f32["value"] / 1e9, 1)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

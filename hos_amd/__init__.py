@@ -40,8 +40,22 @@ EXPORTS = [
     "hos_params_default", "hos_create", "hos_destroy", "hos_footprint", "hos_set_state", "hos_get_state",
     "hos_set_time", "hos_get_time", "hos_step", "hos_step_dt", "hos_get_energy", "hos_rhs", "hos_zvel",
     "hos_fft_2d", "hos_ifft_2d", "hos_sync", "hos_last_step_ms", "hos_use_graph", "hos_set_tile",
-    "hos_last_error", "hos_device_name", "hos_num_members", "hos_bytes_per_point_step",
+    "hos_last_error", "hos_device_name", "hos_num_members", "hos_bytes_per_point_step", "hos_step_times",
+    "hos_profile", "hos_kernel_stats",
 ]
+# include/hos_ic.h
+IC_EXPORTS = ["hos_ic_jonswap", "hos_ic_uniform", "hos_ic_peak_period", "hos_dealias_state"]
+
+KERNEL_KINDS = ["xinv", "yc2r", "yr2c", "yr2c_pair", "xfwdinv", "xfwd_rhs"]
+
+
+class Jonswap(ctypes.Structure):
+    _fields_ = [
+        ("Nx", ctypes.c_int), ("Ny", ctypes.c_int), ("Lx", ctypes.c_double), ("Ly", ctypes.c_double),
+        ("g", ctypes.c_double), ("depth", ctypes.c_double), ("kp", ctypes.c_double),
+        ("gamma", ctypes.c_double), ("eps", ctypes.c_double), ("theta_p", ctypes.c_double),
+        ("kcut", ctypes.c_double), ("seed", ctypes.c_ulonglong),
+    ]
 
 
 def build() -> str:
@@ -85,6 +99,16 @@ def load():
     lib.hos_num_members.argtypes = [vp]
     lib.hos_bytes_per_point_step.argtypes = [P]
     lib.hos_bytes_per_point_step.restype = ctypes.c_double
+    lib.hos_step_times.argtypes = [vp, vp, ip]
+    lib.hos_profile.argtypes = [vp, ip]
+    lib.hos_kernel_stats.argtypes = [vp, ip, dp, ctypes.POINTER(ctypes.c_long), dp]
+    lib.hos_ic_jonswap.argtypes = [ctypes.POINTER(Jonswap), vp]
+    lib.hos_ic_uniform.argtypes = [ctypes.c_ulonglong, ip, ip]
+    lib.hos_ic_uniform.restype = ctypes.c_double
+    lib.hos_ic_peak_period.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    lib.hos_ic_peak_period.restype = ctypes.c_double
+    lib.hos_dealias_state.argtypes = [ip, ip, ip, vp]
+    lib.hos_dealias_state.restype = None
     _lib = lib
     return lib
 
@@ -121,6 +145,24 @@ def footprint(p: Params, nmembers: int = 1) -> int:
 
 def bytes_per_point_step(p: Params) -> float:
     return float(load().hos_bytes_per_point_step(ctypes.byref(p)))
+
+
+def jonswap(Nx, Ny, Lx, Ly, kp=1.0, gamma=3.3, eps=0.1, depth=0.0, g=9.81, theta_p=0.0, kcut=None,
+            seed=0, nlevs=None) -> np.ndarray:
+    """Seeded JONSWAP state [2][Ny/2+1][Nx] (hos_ic_jonswap); dealiased for
+    nlevs when given (main_2d_par.c:258-262)."""
+    lib = load()
+    j = Jonswap(Nx, Ny, Lx, Ly, g, depth, kp, gamma, eps, theta_p, 4 * kp if kcut is None else kcut, seed)
+    u = np.zeros((2, Ny // 2 + 1, Nx), np.complex128)
+    if lib.hos_ic_jonswap(ctypes.byref(j), _ptr(u)) != 0:
+        raise HosError("hos_ic_jonswap rejected its parameters")
+    if nlevs is not None:
+        lib.hos_dealias_state(Nx, Ny, nlevs, _ptr(u))
+    return u
+
+
+def peak_period(g, kp, depth=0.0) -> float:
+    return load().hos_ic_peak_period(g, kp, depth)
 
 
 class Solver:
@@ -206,6 +248,23 @@ class Solver:
         ms = ctypes.c_double()
         _check(self.lib.hos_last_step_ms(self.h, ctypes.byref(ms)))
         return ms.value
+
+    def step_times(self) -> np.ndarray:
+        n = self.lib.hos_step_times(self.h, None, 0)
+        out = np.zeros(max(n, 1))
+        self.lib.hos_step_times(self.h, _ptr(out), n)
+        return out[:n]
+
+    def profile(self, on: bool = True):
+        _check(self.lib.hos_profile(self.h, int(on)))
+
+    def kernel_stats(self) -> dict:
+        out = {}
+        for k, name in enumerate(KERNEL_KINDS):
+            ms, n, by = ctypes.c_double(), ctypes.c_long(), ctypes.c_double()
+            _check(self.lib.hos_kernel_stats(self.h, k, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(by)))
+            out[name] = {"ms": ms.value, "launches": n.value, "bytes": by.value}
+        return out
 
     def use_graph(self, on: bool = True):
         _check(self.lib.hos_use_graph(self.h, int(on)))

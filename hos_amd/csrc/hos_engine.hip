@@ -85,7 +85,70 @@ struct hos_ctx {
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     std::vector<double> host_t;  // host mirror of t
+    // per-step and per-kernel HIP-event timing
+    std::vector<hipEvent_t> step_ev;
+    std::vector<double> step_ms;
+    int prof_on = 0;
+    bool capturing = false;
+    struct ProfRec {
+        int kind;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    std::vector<ProfRec> recs;
+    std::vector<hipEvent_t> evpool;
+    size_t evused = 0;
+    double kms[HOS_K_NKINDS] = {}, kbytes[HOS_K_NKINDS] = {};
+    long kn[HOS_K_NKINDS] = {};
 };
+
+static hipEvent_t pool_event(hos_ctx *c)
+{
+    if (c->evused < c->evpool.size()) return c->evpool[c->evused++];
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    c->evpool.push_back(e);
+    c->evused++;
+    return e;
+}
+
+// Records a HIP event pair around one kernel launch on the context's stream
+// when kernel profiling is on (eager launches only).
+struct Prof {
+    hos_ctx *c;
+    int kind;
+    double bytes;
+    hipEvent_t a = nullptr;
+    Prof(hos_ctx *c_, int k, double by) : c(c_), kind(k), bytes(by)
+    {
+        if (c->prof_on && !c->capturing) {
+            a = pool_event(c);
+            (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~Prof()
+    {
+        if (a) {
+            hipEvent_t b = pool_event(c);
+            (void)hipEventRecord(b, c->stream);
+            c->recs.push_back({kind, a, b, bytes});
+        }
+    }
+};
+
+static void harvest_prof(hos_ctx *c)
+{
+    for (auto &r : c->recs) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            c->kms[r.kind] += ms;
+            c->kbytes[r.kind] += r.bytes;
+            c->kn[r.kind] += 1;
+        }
+    }
+    c->recs.clear();
+    c->evused = 0;
+}
 
 extern "C" void hos_params_default(hos_params *p)
 {
@@ -251,6 +314,8 @@ static void release(hos_ctx *c)
                     c->hpart, c->mean0, c->hostbuf};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
+    for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->step_ev) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -397,6 +462,9 @@ struct Eng {
         LOGN_SWITCH(c->lognx, {
             constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
             dim3 grid((c->Nk + ROWS - 1) / ROWS, nops, nm);
+            int nsrc = 0;
+            for (int q = 0; q < nops; ++q) nsrc |= 1 << ops.o[q].src;
+            Prof pf(c, HOS_K_XINV, (double)nm * c->fs * (16.0 * __builtin_popcount(nsrc) + 2.0 * sizeof(T) * nops));
             hipLaunchKernelGGL((k_xinv<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, Y, sY(), I(), sI(), c->fs, ops,
                                ktab(c), twx());
         });
@@ -408,6 +476,7 @@ struct Eng {
         LOGN_SWITCH(c->logny, {
             TX_SWITCH(T, tx, {
                 dim3 grid(c->p.Nx / TXV, nf, nm);
+                Prof pf(c, HOS_K_YC2R, (double)nm * nf * sizeof(T) * (2.0 * c->fs + c->fsP));
                 hipLaunchKernelGGL((k_yc2r<T, LG, TXV>), grid, dim3(NT), 0, c->stream, I(), sI(), c->fs, P(), sP(),
                                    c->fsP, fm, c->p.Nx, twy());
             })
@@ -415,12 +484,13 @@ struct Eng {
         return HOS_OK;
     }
     template <class PRO>
-    int yr2c(const PRO &pro, int dst)
+    int yr2c(const PRO &pro, int dst, int nin)
     {
         const int tx = c->tx;
         LOGN_SWITCH(c->logny, {
             TX_SWITCH(T, tx, {
                 dim3 grid(c->p.Nx / TXV, 1, nm);
+                Prof pf(c, HOS_K_YR2C, (double)nm * sizeof(T) * (nin * (double)c->fsP + 2.0 * c->fs));
                 hipLaunchKernelGGL((k_yr2c<T, LG, TXV, PRO>), grid, dim3(NT), 0, c->stream, pro, I(), sI(), c->fs,
                                    dst, c->p.Nx, twy());
             })
@@ -428,13 +498,14 @@ struct Eng {
         return HOS_OK;
     }
     template <class PRO>
-    int yr2c_pair(const PRO &pro, int da, int db)
+    int yr2c_pair(const PRO &pro, int da, int db, int nin)
     {
         const int tx = c->tx;
         LOGN_SWITCH(c->logny, {
             TX_SWITCH(T, tx, {
                 constexpr int TXP = TXV / 2;
                 dim3 grid(c->p.Nx / TXP, 1, nm);
+                Prof pf(c, HOS_K_YR2C_PAIR, (double)nm * sizeof(T) * (nin * (double)c->fsP + 4.0 * c->fs));
                 hipLaunchKernelGGL((k_yr2c_pair<T, LG, TXP, PRO>), grid, dim3(NT), 0, c->stream, pro, I(), sI(),
                                    c->fs, da, db, c->p.Nx, twy());
             })
@@ -447,6 +518,7 @@ struct Eng {
         LOGN_SWITCH(c->lognx, {
             constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
             dim3 grid((c->Nk + ROWS - 1) / ROWS, 1, nm);
+            Prof pf(c, HOS_K_XFWDINV, (double)nm * c->fs * 2.0 * sizeof(T) * (1 + ndst));
             hipLaunchKernelGGL((k_xfwdinv<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, I(), sI(), c->fs, src, dst0,
                                ndst, scale, ktab(c), twx());
         });
@@ -455,9 +527,18 @@ struct Eng {
     int xfwd_rhs(int ia, int ib, const cx<double> *Y, const RkArgs &rk)
     {
         const double scale = 1.0 / ((double)c->p.Nx * c->p.Ny);
+        // FP64 [2][Nk][Nx] fields moved by the epilogue
+        double f64 = 0.0;
+        if (rk.mode == EPI_RK_SUBDIAG)
+            f64 = 2 /*Y*/ + 2 /*u*/ + (rk.stage > 0 ? 2 : 0) /*acc*/ + 2 /*acc or u*/ + 2 /*Y next*/;
+        else if (rk.mode == EPI_STORE_K)
+            f64 = 2 + 2 + (rk.hpart && rk.stage == 0 ? 2 : 0);
+        else
+            f64 = 2;
         LOGN_SWITCH(c->lognx, {
             constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
             dim3 grid((c->Nk + ROWS - 1) / ROWS, 1, nm);
+            Prof pf(c, HOS_K_XFWD_RHS, (double)nm * c->fs * (2.0 * 2.0 * sizeof(T) + 16.0 * f64));
             hipLaunchKernelGGL((k_xfwd_rhs<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, I(), sI(), c->fs, ia, ib, Y,
                                sY(), scale, ktab(c), rk, twx());
         });
@@ -488,7 +569,7 @@ struct Eng {
         if (!rc) rc = yc2r(fm, n);
         for (int lev = 1; lev <= M - 1 && !rc; ++lev) {
             ProPhi<T> pro{phys(), ix, lev, comp()};
-            rc = yr2c(pro, 0);
+            rc = yr2c(pro, 0, lev + 1);
             const int nd = M - lev;
             if (!rc) rc = xfwdinv(0, 1, nd);
             FieldMap f2{};
@@ -506,7 +587,7 @@ struct Eng {
         int rc = w_levels(Y);
         if (rc) return rc;
         ProRhs<T> pro{phys(), c->ix, comp()};
-        rc = yr2c_pair(pro, 0, 1);
+        rc = yr2c_pair(pro, 0, 1, c->nP);
         if (!rc) rc = xfwd_rhs(0, 1, Y, rk);
         return rc;
     }
@@ -586,7 +667,9 @@ static int build_graph(hos_ctx *c)
 {
     if (c->gexec) return HOS_OK;
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
     int rc = step_once(c);
+    c->capturing = false;
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc) {
@@ -608,7 +691,13 @@ extern "C" int hos_step_dt(hos_ctx *c, int nsteps, const double *dt)
         int rc = build_graph(c);
         if (rc) return rc;
     }
+    while ((int)c->step_ev.size() < nsteps + 1) {
+        hipEvent_t e = nullptr;
+        HIP_TRY(hipEventCreate(&e));
+        c->step_ev.push_back(e);
+    }
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(hipEventRecord(c->step_ev[0], c->stream));
     for (int n = 0; n < nsteps; ++n) {
         if (c->use_graph) {
             HIP_TRY(hipGraphLaunch(c->gexec, c->stream));
@@ -616,6 +705,7 @@ extern "C" int hos_step_dt(hos_ctx *c, int nsteps, const double *dt)
             int rc = step_once(c);
             if (rc) return rc;
         }
+        HIP_TRY(hipEventRecord(c->step_ev[n + 1], c->stream));
     }
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
     HIP_TRY(hipGetLastError());
@@ -623,6 +713,13 @@ extern "C" int hos_step_dt(hos_ctx *c, int nsteps, const double *dt)
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = nsteps > 0 ? ms / nsteps : 0.0;
+    c->step_ms.resize(nsteps);
+    for (int n = 0; n < nsteps; ++n) {
+        float sm = 0.f;
+        HIP_TRY(hipEventElapsedTime(&sm, c->step_ev[n], c->step_ev[n + 1]));
+        c->step_ms[n] = sm;
+    }
+    harvest_prof(c);
     for (int m = 0; m < c->nm; ++m) c->host_t[m] += nsteps * dt[m];
     return HOS_OK;
 }
@@ -638,6 +735,35 @@ extern "C" int hos_last_step_ms(hos_ctx *c, double *ms)
 {
     if (!c || !ms) return fail(HOS_E_INVALID, "null argument");
     *ms = c->last_ms;
+    return HOS_OK;
+}
+
+extern "C" int hos_step_times(hos_ctx *c, double *ms, int n)
+{
+    if (!c || (!ms && n > 0)) return fail(HOS_E_INVALID, "null argument");
+    const int k = std::min<int>(n, (int)c->step_ms.size());
+    for (int i = 0; i < k; ++i) ms[i] = c->step_ms[i];
+    return (int)c->step_ms.size();
+}
+
+extern "C" int hos_profile(hos_ctx *c, int enable)
+{
+    if (!c) return fail(HOS_E_INVALID, "null context");
+    c->prof_on = enable ? 1 : 0;
+    for (int k = 0; k < HOS_K_NKINDS; ++k) {
+        c->kms[k] = c->kbytes[k] = 0.0;
+        c->kn[k] = 0;
+    }
+    return HOS_OK;
+}
+
+extern "C" int hos_kernel_stats(hos_ctx *c, int kind, double *ms_total, long *launches, double *bytes_total)
+{
+    if (!c) return fail(HOS_E_INVALID, "null context");
+    if (kind < 0 || kind >= HOS_K_NKINDS) return fail(HOS_E_INVALID, "kernel kind %d out of range", kind);
+    if (ms_total) *ms_total = c->kms[kind];
+    if (launches) *launches = c->kn[kind];
+    if (bytes_total) *bytes_total = c->kbytes[kind];
     return HOS_OK;
 }
 
@@ -778,7 +904,7 @@ static int zvel_impl(hos_ctx *c)
     rk.hpart = nullptr;
     for (int which = 0; which < 2 && !rc; ++which) {
         ProZvel<T> pro{e.phys(), c->ix, which, e.comp()};
-        rc = e.yr2c_pair(pro, 0, 1);
+        rc = e.yr2c_pair(pro, 0, 1, c->nP - 4);
         rk.K = c->K + (long)which * 2 * c->fs;
         if (!rc) rc = e.xfwd_rhs(0, 1, c->Y, rk);
     }
@@ -806,7 +932,7 @@ static int fft_impl(hos_ctx *c)
     const long n = c->fsP;
     hipLaunchKernelGGL(k_real_from_f64<T>, dim3(1024), dim3(NT), 0, c->stream, c->hostbuf, e.P(), n);
     ProRead<T> pro{e.phys(), 0};
-    int rc = e.yr2c(pro, 0);
+    int rc = e.yr2c(pro, 0, 1);
     RkArgs rk = rk_base(c);
     rk.mode = EPI_STORE_SPEC2;
     rk.hpart = nullptr;

@@ -125,6 +125,26 @@ int hos_ifft_2d(hos_ctx *c, const double *hu, double *u_padded);
 int hos_sync(hos_ctx *c);
 int hos_last_step_ms(hos_ctx *c, double *ms);
 
+/* Per-step durations (ms, HIP events between steps) of the last hos_step call;
+ * returns the number of steps recorded and copies up to n of them. */
+int hos_step_times(hos_ctx *c, double *ms, int n);
+
+/* Kernel kinds for the per-kernel timing below. */
+#define HOS_K_XINV 0      /* x inverse pass (spectral -> x-intermediate)      */
+#define HOS_K_YC2R 1      /* y c2r pass (x-intermediate -> physical)          */
+#define HOS_K_YR2C 2      /* y r2c pass with a one-field prologue             */
+#define HOS_K_YR2C_PAIR 3 /* y r2c pass with the two-field RHS prologue       */
+#define HOS_K_XFWDINV 4   /* x forward + d^j/dz^j + x inverse (one W level)   */
+#define HOS_K_XFWD_RHS 5  /* x forward pass with the RHS / RK epilogue        */
+#define HOS_K_NKINDS 6
+
+/* 1: record a HIP event pair around every stream-ordered kernel launch of
+ * later hos_step calls (not graph replays) and accumulate, per kernel kind,
+ * the launch count, the summed duration and the algorithmic bytes (the
+ * minimum HBM bytes the launch must read and write).  Resets the totals. */
+int hos_profile(hos_ctx *c, int enable);
+int hos_kernel_stats(hos_ctx *c, int kind, double *ms_total, long *launches, double *bytes_total);
+
 /* Capture the launches of one step into a hipGraph and replay it for every
  * later hos_step of the same dt (1) or launch stream-ordered (0). */
 int hos_use_graph(hos_ctx *c, int enable);

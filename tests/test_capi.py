@@ -21,6 +21,9 @@ def test_library_exports_every_declared_symbol(hos):
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     assert sorted(names) == sorted(hos.EXPORTS)
+    ic = declared("hos_ic.h")
+    assert sorted(ic) == sorted(hos.IC_EXPORTS)
+    assert all(hasattr(lib, n) for n in ic)
 
 
 def test_params_default_and_footprint(hos):
