@@ -41,8 +41,9 @@ EXPORTS = [
     "hos_set_time", "hos_get_time", "hos_step", "hos_step_dt", "hos_get_energy", "hos_rhs", "hos_zvel",
     "hos_fft_2d", "hos_ifft_2d", "hos_sync", "hos_last_step_ms", "hos_use_graph", "hos_set_tile",
     "hos_last_error", "hos_device_name", "hos_num_members", "hos_bytes_per_point_step", "hos_step_times",
-    "hos_profile", "hos_kernel_stats",
+    "hos_profile", "hos_kernel_stats", "hos_spectral_op", "hos_mult", "hos_hamiltonian",
 ]
+HOS_OP_DX, HOS_OP_DY, HOS_OP_DZ, HOS_OP_SUM, HOS_OP_DEALIAS = 1, 2, 3, 4, 5
 # include/hos_ic.h
 IC_EXPORTS = ["hos_ic_jonswap", "hos_ic_uniform", "hos_ic_peak_period", "hos_dealias_state"]
 
@@ -93,7 +94,7 @@ def load():
     lib.hos_sync.argtypes = [vp]
     lib.hos_last_step_ms.argtypes = [vp, dp]
     lib.hos_use_graph.argtypes = [vp, ip]
-    lib.hos_set_tile.argtypes = [vp, ip]
+    lib.hos_set_tile.argtypes = [vp, ip, ip]
     lib.hos_last_error.restype = ctypes.c_char_p
     lib.hos_device_name.argtypes = [ctypes.c_char_p, ip]
     lib.hos_num_members.argtypes = [vp]
@@ -102,6 +103,9 @@ def load():
     lib.hos_step_times.argtypes = [vp, vp, ip]
     lib.hos_profile.argtypes = [vp, ip]
     lib.hos_kernel_stats.argtypes = [vp, ip, dp, ctypes.POINTER(ctypes.c_long), dp]
+    lib.hos_spectral_op.argtypes = [vp, ip, vp, vp, vp]
+    lib.hos_mult.argtypes = [vp, vp, vp, vp]
+    lib.hos_hamiltonian.argtypes = [vp, vp, vp, vp, dp]
     lib.hos_ic_jonswap.argtypes = [ctypes.POINTER(Jonswap), vp]
     lib.hos_ic_uniform.argtypes = [ctypes.c_ulonglong, ip, ip]
     lib.hos_ic_uniform.restype = ctypes.c_double
@@ -249,6 +253,26 @@ class Solver:
         _check(self.lib.hos_last_step_ms(self.h, ctypes.byref(ms)))
         return ms.value
 
+    def spectral_op(self, op: int, a, b=None) -> np.ndarray:
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        bb = None if b is None else np.ascontiguousarray(b, dtype=np.complex128)
+        out = self.spec()
+        _check(self.lib.hos_spectral_op(self.h, op, _ptr(a), None if bb is None else _ptr(bb), _ptr(out)))
+        return out
+
+    def mult(self, a, b) -> np.ndarray:
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        out = self.spec()
+        _check(self.lib.hos_mult(self.h, _ptr(a), _ptr(b), _ptr(out)))
+        return out
+
+    def hamiltonian(self, heta, heta_t, hphi) -> float:
+        a = [np.ascontiguousarray(x, dtype=np.complex128) for x in (heta, heta_t, hphi)]
+        H = ctypes.c_double()
+        _check(self.lib.hos_hamiltonian(self.h, *[_ptr(x) for x in a], ctypes.byref(H)))
+        return H.value
+
     def step_times(self) -> np.ndarray:
         n = self.lib.hos_step_times(self.h, None, 0)
         out = np.zeros(max(n, 1))
@@ -269,8 +293,10 @@ class Solver:
     def use_graph(self, on: bool = True):
         _check(self.lib.hos_use_graph(self.h, int(on)))
 
-    def set_tile(self, tx: int):
-        _check(self.lib.hos_set_tile(self.h, tx))
+    def set_tile(self, tx: int, kind: int = 3):
+        """y-pass tile (4, 8, 16; 0 = default) for kind HOS_TILE_C2R (0),
+        HOS_TILE_R2C (1), HOS_TILE_PAIR (2) or all (3)."""
+        _check(self.lib.hos_set_tile(self.h, kind, tx))
 
     def sync(self):
         _check(self.lib.hos_sync(self.h))

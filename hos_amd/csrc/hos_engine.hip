@@ -60,7 +60,8 @@ constexpr int LOGN_MIN = 4, LOGN_MAX = 11;
 struct hos_ctx {
     hos_params p{};
     int nm = 0, M = 0, L = 0, Nk = 0, lognx = 0, logny = 0, mx = 0, my = 0;
-    int nI = 0, nP = 0, tx = 0;
+    int nI = 0, nP = 0;
+    int tx[3] = {8, 4, 4};  // y tiles: c2r, r2c, pair
     bool f32 = false;
     long fs = 0;    // complex elements per spectral field  (Nk*Nx)
     long fsP = 0;   // real elements per physical field     (Nx*Ny)
@@ -208,17 +209,22 @@ static int setup_scheme(hos_ctx *c, int flg)
     return HOS_OK;
 }
 
-static int default_tx(bool f32, int logny)
+// y-pass tiles: TX x-rows per workgroup, TX/2 complex rows of Ny in LDS
+// (<= 64 KiB).  Wider tiles write longer transposed segments; narrower ones
+// give more workgroups when a launch carries few fields.
+static bool tx_fits(bool f32, int logny, int tx)
 {
-    // keep the y-pass LDS tile at <= 64 KiB: TX/2 rows of Ny complex values
     const int bytes = f32 ? 8 : 16;
-    int tx = 16;
-    while (tx > 4 && (tx / 2) * (1 << logny) * bytes > 65536) tx /= 2;
-    if (!f32 && tx > 8) tx = 8;
-    return tx;
+    return (tx == 4 || tx == 8 || tx == 16) && (tx / 2) * (1 << logny) * bytes <= 65536;
 }
 
-static bool tx_valid(bool f32, int tx) { return f32 ? (tx == 8 || tx == 16) : (tx == 4 || tx == 8); }
+static int default_tx(bool f32, int logny, int kind)
+{
+    // c2r launches carry several fields, r2c ones one (or one pair)
+    int tx = (kind == HOS_TILE_C2R) ? 8 : 4;
+    while (tx > 4 && !tx_fits(f32, logny, tx)) tx /= 2;
+    return tx;
+}
 
 static int validate(const hos_params *p, int nm)
 {
@@ -338,7 +344,7 @@ extern "C" int hos_create(const hos_params *p, int nmembers, hos_ctx **out)
         return rc;
     }
     c->nK = c->subdiag ? 2 : (c->S > 2 ? c->S : 2);
-    c->tx = default_tx(c->f32, c->logny);
+    for (int k = HOS_TILE_C2R; k <= HOS_TILE_PAIR; ++k) c->tx[k] = default_tx(c->f32, c->logny, k);
     const int nm = nmembers;
     auto bail = [&](int code) {
         release(c);
@@ -434,13 +440,22 @@ static KTab ktab(const hos_ctx *c)
     default: return fail(HOS_E_INVALID, "unsupported size 2^%d", v);                                \
     }
 
-#define TX_SWITCH(T, tx, ...)                                                                     \
-    if constexpr (sizeof(T) == 4) {                                                                \
-        if (tx == 16) { constexpr int TXV = 16; __VA_ARGS__; }                                            \
-        else { constexpr int TXV = 8; __VA_ARGS__; }                                                      \
-    } else {                                                                                       \
-        if (tx == 8) { constexpr int TXV = 8; __VA_ARGS__; }                                              \
-        else { constexpr int TXV = 4; __VA_ARGS__; }                                                      \
+// TXV = y tile; only tiles whose LDS fits 64 KiB are instantiated
+#define TX_CASE(T, V, ...)                                                                         \
+    case V: {                                                                                      \
+        constexpr int TXV = V;                                                                     \
+        if constexpr ((TXV / 2) * (1 << LG) * sizeof(cx<T>) <= 65536) {                            \
+            __VA_ARGS__;                                                                           \
+        } else {                                                                                   \
+            return fail(HOS_E_INVALID, "tile %d exceeds 64 KiB of LDS at Ny = %d", V, 1 << LG);     \
+        }                                                                                          \
+    } break;
+#define TX_SWITCH(T, tx, ...)                                                                      \
+    switch (tx) {                                                                                  \
+        TX_CASE(T, 4, __VA_ARGS__)                                                                 \
+        TX_CASE(T, 8, __VA_ARGS__)                                                                 \
+        TX_CASE(T, 16, __VA_ARGS__)                                                                \
+    default: return fail(HOS_E_INVALID, "unsupported tile %d", tx);                                \
     }
 
 template <class T>
@@ -457,14 +472,15 @@ struct Eng {
     Phys<T> phys() const { return Phys<T>{P(), sP(), c->fsP, c->p.Ny}; }
     bool comp() const { return c->f32 && c->p.compensated; }
 
-    int xinv(const cx<double> *Y, const XinvOps &ops, int nops)
+    // ops.n[0] > 0 outputs from source ops.src[0], ops.n[1] >= 0 from ops.src[1]
+    int xinv(const cx<double> *Y, const XinvOps &ops)
     {
+        const int ngroups = ops.n[1] > 0 ? 2 : 1;
+        const int nops = ops.n[0] + ops.n[1];
         LOGN_SWITCH(c->lognx, {
             constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
-            dim3 grid((c->Nk + ROWS - 1) / ROWS, nops, nm);
-            int nsrc = 0;
-            for (int q = 0; q < nops; ++q) nsrc |= 1 << ops.o[q].src;
-            Prof pf(c, HOS_K_XINV, (double)nm * c->fs * (16.0 * __builtin_popcount(nsrc) + 2.0 * sizeof(T) * nops));
+            dim3 grid((c->Nk + ROWS - 1) / ROWS, ngroups, nm);
+            Prof pf(c, HOS_K_XINV, (double)nm * c->fs * (16.0 * ngroups + 2.0 * sizeof(T) * nops));
             hipLaunchKernelGGL((k_xinv<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, Y, sY(), I(), sI(), c->fs, ops,
                                ktab(c), twx());
         });
@@ -472,7 +488,7 @@ struct Eng {
     }
     int yc2r(const FieldMap &fm, int nf)
     {
-        const int tx = c->tx;
+        const int tx = c->tx[HOS_TILE_C2R];
         LOGN_SWITCH(c->logny, {
             TX_SWITCH(T, tx, {
                 dim3 grid(c->p.Nx / TXV, nf, nm);
@@ -486,7 +502,7 @@ struct Eng {
     template <class PRO>
     int yr2c(const PRO &pro, int dst, int nin)
     {
-        const int tx = c->tx;
+        const int tx = c->tx[HOS_TILE_R2C];
         LOGN_SWITCH(c->logny, {
             TX_SWITCH(T, tx, {
                 dim3 grid(c->p.Nx / TXV, 1, nm);
@@ -500,7 +516,7 @@ struct Eng {
     template <class PRO>
     int yr2c_pair(const PRO &pro, int da, int db, int nin)
     {
-        const int tx = c->tx;
+        const int tx = c->tx[HOS_TILE_PAIR];
         LOGN_SWITCH(c->logny, {
             TX_SWITCH(T, tx, {
                 constexpr int TXP = TXV / 2;
@@ -553,8 +569,10 @@ struct Eng {
         XinvOps ops{};
         FieldMap fm{};
         int n = 0;
+        ops.src[0] = 0;
+        ops.src[1] = 1;
         auto add = [&](int src, int op, int pw, int pdst) {
-            ops.o[n] = {src, op, pw, n};
+            ops.o[src][ops.n[src]++] = {src, op, pw, n};
             fm.src[n] = n;
             fm.dst[n] = pdst;
             ++n;
@@ -565,7 +583,7 @@ struct Eng {
         add(1, OP_DX, 0, ix.phix);
         add(1, OP_DY, 0, ix.phiy);
         for (int j = 1; j <= M; ++j) add(1, OP_DZ, j, ix.d[1][j]);
-        int rc = xinv(Y, ops, n);
+        int rc = xinv(Y, ops);
         if (!rc) rc = yc2r(fm, n);
         for (int lev = 1; lev <= M - 1 && !rc; ++lev) {
             ProPhi<T> pro{phys(), ix, lev, comp()};
@@ -774,15 +792,19 @@ extern "C" int hos_sync(hos_ctx *c)
     return HOS_OK;
 }
 
-extern "C" int hos_set_tile(hos_ctx *c, int tx)
+extern "C" int hos_set_tile(hos_ctx *c, int kind, int tx)
 {
     if (!c) return fail(HOS_E_INVALID, "null context");
-    if (tx == 0) tx = default_tx(c->f32, c->logny);
-    if (!tx_valid(c->f32, tx)) return fail(HOS_E_INVALID, "tile %d not available for this precision", tx);
-    if (c->p.Nx % tx) return fail(HOS_E_INVALID, "tile %d does not divide Nx", tx);
-    const int bytes = c->f32 ? 8 : 16;
-    if ((tx / 2) * (1 << c->logny) * bytes > 65536) return fail(HOS_E_INVALID, "tile %d exceeds 64 KiB LDS", tx);
-    c->tx = tx;
+    if (kind < HOS_TILE_C2R || kind > HOS_TILE_ALL) return fail(HOS_E_INVALID, "unknown tile kind %d", kind);
+    for (int k = HOS_TILE_C2R; k <= HOS_TILE_PAIR; ++k) {
+        if (kind != HOS_TILE_ALL && kind != k) continue;
+        const int v = tx ? tx : default_tx(c->f32, c->logny, k);
+        if (!tx_fits(c->f32, c->logny, v))
+            return fail(HOS_E_INVALID, "tile %d not available (4, 8 or 16 rows, <= 64 KiB LDS)", v);
+        if (c->p.Nx % v) return fail(HOS_E_INVALID, "tile %d does not divide Nx", v);
+    }
+    for (int k = HOS_TILE_C2R; k <= HOS_TILE_PAIR; ++k)
+        if (kind == HOS_TILE_ALL || kind == k) c->tx[k] = tx ? tx : default_tx(c->f32, c->logny, k);
     if (c->gexec) {
         (void)hipGraphExecDestroy(c->gexec);
         (void)hipGraphDestroy(c->graph);
@@ -960,11 +982,12 @@ static int ifft_impl(hos_ctx *c)
 {
     Eng<T> e{c, 1};
     XinvOps ops{};
-    ops.o[0] = {0, OP_ID, 0, 0};
+    ops.o[0][0] = {0, OP_ID, 0, 0};
+    ops.n[0] = 1;
     FieldMap fm{};
     fm.src[0] = 0;
     fm.dst[0] = 0;
-    int rc = e.xinv(c->Y, ops, 1);
+    int rc = e.xinv(c->Y, ops);
     if (!rc) rc = e.yc2r(fm, 1);
     hipLaunchKernelGGL(k_real_to_f64<T>, dim3(1024), dim3(NT), 0, c->stream, e.P(), c->hostbuf, c->fsP);
     return rc;
@@ -983,6 +1006,86 @@ extern "C" int hos_ifft_2d(hos_ctx *c, const double *hu, double *u_padded)
     HIP_TRY(hipStreamSynchronize(c->stream));
     for (int x = 0; x < Nx; ++x) u_padded[(size_t)x * (Ny + 2) + Ny] = u_padded[(size_t)x * (Ny + 2) + Ny + 1] = 0.0;
     return restore_member0(c);
+}
+
+// Mult (operators_par.c:210-235): c2r of a and b, product on the grid, r2c.
+template <class T>
+static int mult_impl(hos_ctx *c)
+{
+    Eng<T> e{c, 1};
+    XinvOps ops{};
+    ops.o[0][0] = {0, OP_ID, 0, 0};
+    ops.o[1][0] = {1, OP_ID, 0, 1};
+    ops.n[0] = ops.n[1] = 1;
+    ops.src[0] = 0;
+    ops.src[1] = 1;
+    FieldMap fm{};
+    fm.src[0] = 0;
+    fm.dst[0] = 0;
+    fm.src[1] = 1;
+    fm.dst[1] = 1;
+    int rc = e.xinv(c->Y, ops);
+    if (!rc) rc = e.yc2r(fm, 2);
+    ProMul<T> pro{e.phys(), 0, 1};
+    if (!rc) rc = e.yr2c(pro, 0, 2);
+    RkArgs rk = rk_base(c);
+    rk.mode = EPI_STORE_SPEC2;
+    rk.hpart = nullptr;
+    rk.K = c->K;
+    if (!rc) rc = e.xfwd_rhs(0, 0, c->Y, rk);
+    return rc;
+}
+
+extern "C" int hos_mult(hos_ctx *c, const double *a, const double *b, double *out)
+{
+    if (!c || !a || !b || !out) return fail(HOS_E_INVALID, "null argument");
+    const size_t n = c->fs * sizeof(cx<double>);
+    HIP_TRY(hipMemcpyAsync(c->Y, a, n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->Y + c->fs, b, n, hipMemcpyHostToDevice, c->stream));
+    int rc = c->f32 ? mult_impl<float>(c) : mult_impl<double>(c);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(out, c->K, n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipGetLastError());
+    return restore_member0(c);
+}
+
+extern "C" int hos_spectral_op(hos_ctx *c, int op, const double *a, const double *b, double *out)
+{
+    if (!c || !a || !out) return fail(HOS_E_INVALID, "null argument");
+    if (op < HOS_OP_DX || op > HOS_OP_DEALIAS) return fail(HOS_E_INVALID, "unknown spectral op %d", op);
+    if (op == HOS_OP_SUM && !b) return fail(HOS_E_INVALID, "sum needs two inputs");
+    const size_t n = c->fs * sizeof(cx<double>);
+    HIP_TRY(hipMemcpyAsync(c->K, a, n, hipMemcpyHostToDevice, c->stream));
+    if (b) HIP_TRY(hipMemcpyAsync(c->K + c->fs, b, n, hipMemcpyHostToDevice, c->stream));
+    const int grid = (int)std::min<long>((c->fs + NT - 1) / NT, 1024);
+    hipLaunchKernelGGL(k_spec_op, dim3(grid), dim3(NT), 0, c->stream, c->K, c->K + c->fs, c->K + 2 * c->fs, op,
+                       c->p.Nx, ktab(c));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, c->K + 2 * c->fs, n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return HOS_OK;
+}
+
+extern "C" int hos_hamiltonian(hos_ctx *c, const double *heta, const double *heta_t, const double *hphi,
+                               double *H)
+{
+    if (!c || !heta || !heta_t || !hphi || !H) return fail(HOS_E_INVALID, "null argument");
+    const size_t n = c->fs * sizeof(cx<double>);
+    HIP_TRY(hipMemcpyAsync(c->K, heta, n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->K + c->fs, heta_t, n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->K + 2 * c->fs, hphi, n, hipMemcpyHostToDevice, c->stream));
+    const int nparts = (c->Nk + xrows(c->lognx) - 1) / xrows(c->lognx) * c->nm;
+    const int grid = std::min(nparts, 64);
+    hipLaunchKernelGGL(k_hamiltonian, dim3(grid), dim3(NT), 0, c->stream, c->K, c->K + c->fs, c->K + 2 * c->fs,
+                       c->p.g, c->p.Nx, c->Nk, c->hpart);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> part(grid);
+    HIP_TRY(hipMemcpyAsync(part.data(), c->hpart, sizeof(double) * grid, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double s = 0.0;
+    for (double v : part) s += v;
+    *H = s;
+    return HOS_OK;
 }
 
 // ---------------------------------------------------------------------------

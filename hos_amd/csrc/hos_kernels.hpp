@@ -57,8 +57,11 @@ struct XinvOp {
     int pow;  // d^pow/dz^pow for OP_DZ
     int dst;  // intermediate index
 };
+// outputs grouped by source field: group g reads its source row once
 struct XinvOps {
-    XinvOp o[MAXOPS];
+    XinvOp o[2][MAXOPS];
+    int n[2];
+    int src[2];
 };
 
 // XCD-aware tile order: the 8 XCDs take workgroups round-robin, so give
@@ -71,47 +74,65 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
 }
 
 // ---------------------------------------------------------------------------
-// x inverse pass: spectral (FP64 stage input, with multiplier) -> I.
-// grid (ceil(Nk/ROWS), nops, members)
+// x inverse pass: spectral (FP64 stage input, with multiplier) -> I.  The
+// source row is read once into registers (FP64) and every output of the
+// group is formed from it.  grid (ceil(Nk/ROWS), ngroups, members)
 template <class T, int LOGNX, int ROWS>
 __global__ __launch_bounds__(NT) void k_xinv(const cx<double> *__restrict__ Y, long sY, cx<T> *__restrict__ I,
                                              long sI, long fs, XinvOps ops, KTab kt,
                                              const cx<T> *__restrict__ twx)
 {
     constexpr int NX = 1 << LOGNX;
+    constexpr int PER = (ROWS * NX + NT - 1) / NT;
     __shared__ cx<T> buf[ROWS * NX];
-    const XinvOp op = ops.o[blockIdx.y];
+    const int grp = blockIdx.y;
     const int ky0 = blockIdx.x * ROWS;
-    const cx<double> *src = Y + blockIdx.z * sY + op.src * fs;
-    for (int e = threadIdx.x; e < ROWS * NX; e += NT) {
+    const cx<double> *src = Y + blockIdx.z * sY + ops.src[grp] * fs;
+    cx<double> s[PER];
+    double kxv[PER], kz[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = threadIdx.x + q * NT;
         const int r = e / NX, kx = e % NX, ky = ky0 + r;
-        cx<T> v = {T(0), T(0)};
-        if (ky < kt.nk) {
-            const cx<double> s = src[(long)ky * NX + kx];
-            double re = s.x, im = s.y;
+        s[q] = {0.0, 0.0};
+        kxv[q] = 0.0;
+        kz[q] = 0.0;
+        if (e < ROWS * NX && ky < kt.nk) {
+            s[q] = src[(long)ky * NX + kx];
+            kxv[q] = kt.kx[kx];
+            const double kyv = ky * kt.ky0;
+            kz[q] = sqrt(kxv[q] * kxv[q] + kyv * kyv);
+        }
+    }
+    cx<T> *base = I + blockIdx.z * sI;
+    for (int o = 0; o < ops.n[grp]; ++o) {
+        const XinvOp op = ops.o[grp][o];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = threadIdx.x + q * NT;
+            if (e >= ROWS * NX) break;
+            const int ky = ky0 + e / NX;
+            double re = s[q].x, im = s[q].y;
             if (op.op == OP_DX) {
-                const double k = kt.kx[kx];
-                re = -k * s.y;
-                im = k * s.x;
+                re = -kxv[q] * s[q].y;
+                im = kxv[q] * s[q].x;
             } else if (op.op == OP_DY) {
                 const double k = ky * kt.ky0;
-                re = -k * s.y;
-                im = k * s.x;
+                re = -k * s[q].y;
+                im = k * s[q].x;
             } else if (op.op == OP_DZ) {
-                const double kxv = kt.kx[kx], kyv = ky * kt.ky0;
-                const double m = dz_mul(sqrt(kxv * kxv + kyv * kyv), op.pow, kt.depth);
-                re = m * s.x;
-                im = m * s.y;
+                const double m = dz_mul(kz[q], op.pow, kt.depth);
+                re = m * s[q].x;
+                im = m * s[q].y;
             }
-            v = {T(re), T(im)};
+            buf[e] = {T(re), T(im)};
         }
-        buf[e] = v;
-    }
-    lds_fft<T, LOGNX, ROWS, NT, true>(buf, twx);
-    cx<T> *dst = I + blockIdx.z * sI + op.dst * fs;
-    for (int e = threadIdx.x; e < ROWS * NX; e += NT) {
-        const int r = e / NX, kx = e % NX, ky = ky0 + r;
-        if (ky < kt.nk) dst[(long)ky * NX + kx] = buf[e];
+        lds_fft<T, LOGNX, ROWS, NT, true>(buf, twx);
+        cx<T> *dst = base + op.dst * fs;
+        for (int e = threadIdx.x; e < ROWS * NX; e += NT) {
+            const int r = e / NX, kx = e % NX, ky = ky0 + r;
+            if (ky < kt.nk) dst[(long)ky * NX + kx] = buf[e];
+        }
     }
 }
 
@@ -588,6 +609,72 @@ __global__ __launch_bounds__(NT) void k_combine(const cx<double> *__restrict__ U
         out[m * sM + i] = v;
     }
 }
+
+// pointwise spectral operators on FP64 fields [Nk][Nx]
+// (operators_par.c: Dx 50-71, Dy 105-116, Dz 168-192, Sum 266-276, Dealias 309-322)
+enum : int { SOP_DX = 1, SOP_DY = 2, SOP_DZ = 3, SOP_SUM = 4, SOP_DEALIAS = 5 };
+__global__ __launch_bounds__(NT) void k_spec_op(const cx<double> *__restrict__ a, const cx<double> *__restrict__ b,
+                                                cx<double> *__restrict__ out, int op, int nx, KTab kt)
+{
+    const long n = (long)kt.nk * nx;
+    for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+        const int kx = (int)(i % nx), ky = (int)(i / nx);
+        const cx<double> s = a[i];
+        cx<double> r = s;
+        if (op == SOP_DX) {
+            const double k = kt.kx[kx];
+            r = {-k * s.y, k * s.x};
+        } else if (op == SOP_DY) {
+            const double k = ky * kt.ky0;
+            r = {-k * s.y, k * s.x};
+        } else if (op == SOP_DZ) {
+            const double kxv = kt.kx[kx], kyv = ky * kt.ky0;
+            const double m = dz_mul(sqrt(kxv * kxv + kyv * kyv), 1, kt.depth);
+            r = {m * s.x, m * s.y};
+        } else if (op == SOP_SUM) {
+            r = {s.x + b[i].x, s.y + b[i].y};
+        } else if (op == SOP_DEALIAS) {
+            if (masked(kx, ky, nx, kt.mx, kt.my)) r = {0.0, 0.0};
+        }
+        out[i] = r;
+    }
+}
+
+// Hamiltonian of three FP64 spectral fields (hosm_2d_par.c:462-523):
+// partial sums per block, row j=0 weight 1, others 2.
+__global__ __launch_bounds__(NT) void k_hamiltonian(const cx<double> *__restrict__ eta,
+                                                    const cx<double> *__restrict__ etat,
+                                                    const cx<double> *__restrict__ phi, double g, int nx, int nk,
+                                                    double *part)
+{
+    __shared__ double red[NT / 64];
+    const long n = (long)nk * nx;
+    double h = 0.0;
+    for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+        const double w = (i < nx) ? 1.0 : 2.0;
+        const cx<double> e = eta[i], t = etat[i], p = phi[i];
+        h += w * (g * (e.x * e.x + e.y * e.y) + p.x * t.x + p.y * t.y);
+    }
+    for (int off = 32; off > 0; off >>= 1) h += __shfl_down(h, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = h;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < NT / 64; ++w) s += red[w];
+        part[blockIdx.x] = s;
+    }
+}
+
+// product of two physical fields (the pointwise step of Mult, operators_par.c:222-231)
+template <class T>
+struct ProMul {
+    Phys<T> ph;
+    int fa, fb;
+    __device__ __forceinline__ T operator()(int m, int x, int y) const
+    {
+        return ph.at(m, fa, x, y) * ph.at(m, fb, x, y);
+    }
+};
 
 // FP64 spectral field [Nk][Nx] (row stride nx) copy, optionally from a
 // different source; used for state set-up on the device.

@@ -120,6 +120,28 @@ int hos_zvel(hos_ctx *c, const double *u, double *wM, double *wM2, double *w2M, 
 int hos_fft_2d(hos_ctx *c, const double *u_padded, double *hu);
 int hos_ifft_2d(hos_ctx *c, const double *hu, double *u_padded);
 
+/* The reference's spectral operators on host fields [Ny/2+1][Nx], run on the
+ * GPU with the context's wavenumber table and mask:
+ *   HOS_OP_DX  i kx a            (Dx, operators_par.c:20-75)
+ *   HOS_OP_DY  i ky a            (Dy, :85-120)
+ *   HOS_OP_DZ  |k| a, or |k| tanh(|k| h) a at finite depth (Dz, :130-196)
+ *   HOS_OP_SUM a + b             (Sum, :247-279)
+ *   HOS_OP_DEALIAS  mask of a    (Dealias, :283-325)
+ * b is used by HOS_OP_SUM only; out may alias a or b. */
+#define HOS_OP_DX 1
+#define HOS_OP_DY 2
+#define HOS_OP_DZ 3
+#define HOS_OP_SUM 4
+#define HOS_OP_DEALIAS 5
+int hos_spectral_op(hos_ctx *c, int op, const double *a, const double *b, double *out);
+
+/* Mult (operators_par.c:210-235): pseudo-spectral product of a and b; out may
+ * alias either input. */
+int hos_mult(hos_ctx *c, const double *a, const double *b, double *out);
+
+/* Hamiltonian (hosm_2d_par.c:462-523) of host fields, FP64, on the GPU. */
+int hos_hamiltonian(hos_ctx *c, const double *heta, const double *heta_t, const double *hphi, double *H);
+
 /* Device synchronisation and per-step timing of the last hos_step call
  * (milliseconds per step from HIP events). */
 int hos_sync(hos_ctx *c);
@@ -149,9 +171,14 @@ int hos_kernel_stats(hos_ctx *c, int kind, double *ms_total, long *launches, dou
  * later hos_step of the same dt (1) or launch stream-ordered (0). */
 int hos_use_graph(hos_ctx *c, int enable);
 
-/* Launch-shape overrides chosen by the autotuner (0 = built-in default):
- * tx = x-rows per y-pass workgroup (4, 8 or 16). */
-int hos_set_tile(hos_ctx *c, int tx);
+/* Launch-shape overrides chosen by the autotuner: tx = x-rows per y-pass
+ * workgroup (4, 8 or 16, at most 64 KiB of LDS; 0 = built-in default) for
+ * one kind of y pass or for all of them. */
+#define HOS_TILE_C2R 0   /* k_yc2r: several fields per launch        */
+#define HOS_TILE_R2C 1   /* k_yr2c: one field per launch              */
+#define HOS_TILE_PAIR 2  /* k_yr2c_pair: the two RHS fields            */
+#define HOS_TILE_ALL 3
+int hos_set_tile(hos_ctx *c, int kind, int tx);
 
 const char *hos_last_error(void);
 
