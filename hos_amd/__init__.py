@@ -42,6 +42,7 @@ EXPORTS = [
     "hos_fft_2d", "hos_ifft_2d", "hos_sync", "hos_last_step_ms", "hos_use_graph", "hos_set_tile",
     "hos_last_error", "hos_device_name", "hos_num_members", "hos_bytes_per_point_step", "hos_step_times",
     "hos_profile", "hos_kernel_stats", "hos_spectral_op", "hos_mult", "hos_hamiltonian",
+    "hos_set_active", "hos_get_active", "hos_set_member", "hos_copy_member", "hos_step_async", "hos_wait",
 ]
 HOS_OP_DX, HOS_OP_DY, HOS_OP_DZ, HOS_OP_SUM, HOS_OP_DEALIAS = 1, 2, 3, 4, 5
 # include/hos_ic.h
@@ -106,6 +107,12 @@ def load():
     lib.hos_spectral_op.argtypes = [vp, ip, vp, vp, vp]
     lib.hos_mult.argtypes = [vp, vp, vp, vp]
     lib.hos_hamiltonian.argtypes = [vp, vp, vp, vp, dp]
+    lib.hos_set_active.argtypes = [vp, ip]
+    lib.hos_get_active.argtypes = [vp]
+    lib.hos_set_member.argtypes = [vp, ip, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    lib.hos_copy_member.argtypes = [vp, ip, ip]
+    lib.hos_step_async.argtypes = [vp, ip]
+    lib.hos_wait.argtypes = [vp]
     lib.hos_ic_jonswap.argtypes = [ctypes.POINTER(Jonswap), vp]
     lib.hos_ic_uniform.argtypes = [ctypes.c_ulonglong, ip, ip]
     lib.hos_ic_uniform.restype = ctypes.c_double
@@ -252,6 +259,21 @@ class Solver:
         ms = ctypes.c_double()
         _check(self.lib.hos_last_step_ms(self.h, ctypes.byref(ms)))
         return ms.value
+
+    def set_active(self, n: int):
+        _check(self.lib.hos_set_active(self.h, n))
+
+    def set_member(self, m: int, depth: float, dt: float, t: float = 0.0):
+        _check(self.lib.hos_set_member(self.h, m, depth, dt, t))
+
+    def copy_member(self, src: int, dst: int):
+        _check(self.lib.hos_copy_member(self.h, src, dst))
+
+    def step_async(self, nsteps: int):
+        _check(self.lib.hos_step_async(self.h, nsteps))
+
+    def wait(self):
+        _check(self.lib.hos_wait(self.h))
 
     def spectral_op(self, op: int, a, b=None) -> np.ndarray:
         a = np.ascontiguousarray(a, dtype=np.complex128)

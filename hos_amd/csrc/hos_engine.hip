@@ -71,7 +71,12 @@ struct hos_ctx {
     void *I = nullptr, *P = nullptr;
     void *twx = nullptr, *twy = nullptr;
     double *kx = nullptr, *t = nullptr, *dt = nullptr, *H = nullptr, *vol = nullptr, *hpart = nullptr,
-           *mean0 = nullptr;
+           *mean0 = nullptr, *depth = nullptr;
+    int active = 0;  // members 0..active-1 are stepped
+    std::vector<hipGraphExec_t> gexec_by_active;
+    std::vector<hipGraph_t> graph_by_active;
+    bool async_pending = false;
+    int async_steps = 0;
     double *hostbuf = nullptr;  // device staging [Nx][Ny] double
     int nK = 0;                 // number of FP64 [2][Nk][Nx] slots in K (per member for RK)
     HosIdx ix{};
@@ -83,9 +88,8 @@ struct hos_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     int use_graph = 0;
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
-    std::vector<double> host_t;  // host mirror of t
+    std::vector<double> host_t;   // host mirror of t
+    std::vector<double> host_dt;  // host mirror of dt
     // per-step and per-kernel HIP-event timing
     std::vector<hipEvent_t> step_ev;
     std::vector<double> step_ms;
@@ -312,12 +316,13 @@ static int upload_twiddles(int n, void **dst)
     return HOS_OK;
 }
 
+static void drop_graphs(hos_ctx *c);
+
 static void release(hos_ctx *c)
 {
-    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-    if (c->graph) (void)hipGraphDestroy(c->graph);
+    drop_graphs(c);
     void *ptrs[] = {c->U, c->A, c->Y, c->K, c->I, c->P, c->twx, c->twy, c->kx, c->t, c->dt, c->H, c->vol,
-                    c->hpart, c->mean0, c->hostbuf};
+                    c->hpart, c->mean0, c->depth, c->hostbuf};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
@@ -375,6 +380,7 @@ extern "C" int hos_create(const hos_params *p, int nmembers, hos_ctx **out)
                   {(void **)&c->H, sizeof(double) * nm},
                   {(void **)&c->vol, sizeof(double) * nm},
                   {(void **)&c->mean0, sizeof(double) * nm},
+                  {(void **)&c->depth, sizeof(double) * nm},
                   {(void **)&c->hpart, sizeof(double) * nm * nparts},
                   {(void **)&c->hostbuf, sizeof(double) * c->fsP}};
     for (auto &al : allocs) {
@@ -397,6 +403,15 @@ extern "C" int hos_create(const hos_params *p, int nmembers, hos_ctx **out)
     if (!rc) rc = c->f32 ? upload_twiddles<float>(p->Ny, &c->twy) : upload_twiddles<double>(p->Ny, &c->twy);
     if (rc) return bail(rc);
     c->host_t.assign(nm, 0.0);
+    c->host_dt.assign(nm, 0.0);
+    c->active = nm;
+    {
+        std::vector<double> dep(nm, p->depth);
+        if (hipMemcpy(c->depth, dep.data(), sizeof(double) * nm, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(HOS_E_DEVICE, "depth upload failed"));
+    }
+    c->gexec_by_active.assign(nm + 1, nullptr);
+    c->graph_by_active.assign(nm + 1, nullptr);
     if (hipDeviceSynchronize() != hipSuccess) return bail(fail(HOS_E_DEVICE, "device sync failed"));
     *out = c;
     return HOS_OK;
@@ -420,7 +435,7 @@ static KTab ktab(const hos_ctx *c)
     KTab k;
     k.kx = c->kx;
     k.ky0 = 2 * M_PI / c->p.Ly;
-    k.depth = c->p.depth;
+    k.depth = c->depth;
     k.nk = c->Nk;
     k.mx = c->mx;
     k.my = c->my;
@@ -635,7 +650,7 @@ static RkArgs rk_base(hos_ctx *c)
 template <class T>
 static int launch_step(hos_ctx *c)
 {
-    Eng<T> e{c, c->nm};
+    Eng<T> e{c, c->active};
     const int nparts = (c->Nk + xrows(c->lognx) - 1) / xrows(c->lognx);
     int rc = HOS_OK;
     for (int s = 0; s < c->S && !rc; ++s) {
@@ -656,19 +671,19 @@ static int launch_step(hos_ctx *c)
             ca.np = s + 1;
             for (int q = 0; q <= s; ++q) ca.coef[q] = last ? c->c[q] : c->b[s + 1][q];
             const long n = 2 * c->fs;
-            dim3 grid((unsigned)std::min<long>((n + NT - 1) / NT, 2048), c->nm);
+            dim3 grid((unsigned)std::min<long>((n + NT - 1) / NT, 2048), c->active);
             if (!rc) {
                 hipLaunchKernelGGL(k_combine, grid, dim3(NT), 0, c->stream, c->U, c->K, (long)c->nm * n, n,
                                    last ? c->U : c->Y, n, c->dt, ca);
                 if (last)
                     hipLaunchKernelGGL(k_copy_c64, dim3(2048), dim3(NT), 0, c->stream, c->U, c->Y,
-                                       (long)c->nm * n);
+                                       (long)c->active * n);
             }
         }
     }
     if (!rc)
-        hipLaunchKernelGGL(k_end_step, dim3(c->nm), dim3(NT), 0, c->stream, c->hpart, nparts, c->H, c->t, c->dt,
-                           1);
+        hipLaunchKernelGGL(k_end_step, dim3(c->active), dim3(NT), 0, c->stream, c->hpart, nparts, c->H, c->t,
+                           c->dt, 1);
     return rc;
 }
 
@@ -681,9 +696,20 @@ extern "C" int hos_use_graph(hos_ctx *c, int enable)
     return HOS_OK;
 }
 
+static void drop_graphs(hos_ctx *c)
+{
+    for (size_t i = 0; i < c->gexec_by_active.size(); ++i) {
+        if (c->gexec_by_active[i]) (void)hipGraphExecDestroy(c->gexec_by_active[i]);
+        if (c->graph_by_active[i]) (void)hipGraphDestroy(c->graph_by_active[i]);
+        c->gexec_by_active[i] = nullptr;
+        c->graph_by_active[i] = nullptr;
+    }
+}
+
+// one captured step per active-member count (grid sizes are baked in)
 static int build_graph(hos_ctx *c)
 {
-    if (c->gexec) return HOS_OK;
+    if (c->gexec_by_active[c->active]) return HOS_OK;
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
     int rc = step_once(c);
@@ -695,17 +721,17 @@ static int build_graph(hos_ctx *c)
         return rc;
     }
     if (e != hipSuccess) return fail(HOS_E_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
-    c->graph = g;
-    HIP_TRY(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+    c->graph_by_active[c->active] = g;
+    HIP_TRY(hipGraphInstantiate(&c->gexec_by_active[c->active], g, nullptr, nullptr, 0));
     return HOS_OK;
 }
 
-extern "C" int hos_step_dt(hos_ctx *c, int nsteps, const double *dt)
+// enqueue nsteps steps of the active members with the dt already on the device
+static int step_enqueue(hos_ctx *c, int nsteps)
 {
-    if (!c || !dt) return fail(HOS_E_INVALID, "null argument");
+    if (c->async_pending) return fail(HOS_E_STATE, "a previous hos_step_async has not been waited for");
     if (nsteps < 0) return fail(HOS_E_INVALID, "nsteps < 0");
-    HIP_TRY(hipMemcpyAsync(c->dt, dt, sizeof(double) * c->nm, hipMemcpyHostToDevice, c->stream));
-    if (c->use_graph) {
+    if (c->use_graph && c->active > 0) {
         int rc = build_graph(c);
         if (rc) return rc;
     }
@@ -716,9 +742,9 @@ extern "C" int hos_step_dt(hos_ctx *c, int nsteps, const double *dt)
     }
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
     HIP_TRY(hipEventRecord(c->step_ev[0], c->stream));
-    for (int n = 0; n < nsteps; ++n) {
+    for (int n = 0; n < nsteps && c->active > 0; ++n) {
         if (c->use_graph) {
-            HIP_TRY(hipGraphLaunch(c->gexec, c->stream));
+            HIP_TRY(hipGraphLaunch(c->gexec_by_active[c->active], c->stream));
         } else {
             int rc = step_once(c);
             if (rc) return rc;
@@ -727,6 +753,16 @@ extern "C" int hos_step_dt(hos_ctx *c, int nsteps, const double *dt)
     }
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
     HIP_TRY(hipGetLastError());
+    c->async_pending = true;
+    c->async_steps = c->active > 0 ? nsteps : 0;
+    return HOS_OK;
+}
+
+static int step_finish(hos_ctx *c)
+{
+    if (!c->async_pending) return HOS_OK;
+    c->async_pending = false;
+    const int nsteps = c->async_steps;
     HIP_TRY(hipEventSynchronize(c->ev1));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -738,8 +774,24 @@ extern "C" int hos_step_dt(hos_ctx *c, int nsteps, const double *dt)
         c->step_ms[n] = sm;
     }
     harvest_prof(c);
-    for (int m = 0; m < c->nm; ++m) c->host_t[m] += nsteps * dt[m];
+    for (int m = 0; m < c->active; ++m) c->host_t[m] += nsteps * c->host_dt[m];
     return HOS_OK;
+}
+
+static int upload_dt(hos_ctx *c, const double *dt)
+{
+    for (int m = 0; m < c->nm; ++m) c->host_dt[m] = dt[m];
+    HIP_TRY(hipMemcpy(c->dt, c->host_dt.data(), sizeof(double) * c->nm, hipMemcpyHostToDevice));
+    return HOS_OK;
+}
+
+extern "C" int hos_step_dt(hos_ctx *c, int nsteps, const double *dt)
+{
+    if (!c || !dt) return fail(HOS_E_INVALID, "null argument");
+    int rc = upload_dt(c, dt);
+    if (!rc) rc = step_enqueue(c, nsteps);
+    if (!rc) rc = step_finish(c);
+    return rc;
 }
 
 extern "C" int hos_step(hos_ctx *c, int nsteps, double dt)
@@ -747,6 +799,60 @@ extern "C" int hos_step(hos_ctx *c, int nsteps, double dt)
     if (!c) return fail(HOS_E_INVALID, "null context");
     std::vector<double> d(c->nm, dt);
     return hos_step_dt(c, nsteps, d.data());
+}
+
+extern "C" int hos_step_async(hos_ctx *c, int nsteps)
+{
+    if (!c) return fail(HOS_E_INVALID, "null context");
+    return step_enqueue(c, nsteps);
+}
+
+extern "C" int hos_wait(hos_ctx *c)
+{
+    if (!c) return fail(HOS_E_INVALID, "null context");
+    return step_finish(c);
+}
+
+extern "C" int hos_set_active(hos_ctx *c, int n)
+{
+    if (!c) return fail(HOS_E_INVALID, "null context");
+    if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
+    if (n < 0 || n > c->nm) return fail(HOS_E_INVALID, "active count %d out of [0, %d]", n, c->nm);
+    c->active = n;
+    return HOS_OK;
+}
+
+extern "C" int hos_get_active(hos_ctx *c) { return c ? c->active : 0; }
+
+extern "C" int hos_set_member(hos_ctx *c, int m, double depth, double dt, double t)
+{
+    if (!c) return fail(HOS_E_INVALID, "null context");
+    if (m < 0 || m >= c->nm) return fail(HOS_E_INVALID, "member %d out of range [0, %d)", m, c->nm);
+    if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
+    c->host_dt[m] = dt;
+    c->host_t[m] = t;
+    HIP_TRY(hipMemcpy(c->depth + m, &depth, sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->dt + m, &c->host_dt[m], sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->t + m, &c->host_t[m], sizeof(double), hipMemcpyHostToDevice));
+    return HOS_OK;
+}
+
+extern "C" int hos_copy_member(hos_ctx *c, int src, int dst)
+{
+    if (!c) return fail(HOS_E_INVALID, "null context");
+    if (src < 0 || src >= c->nm || dst < 0 || dst >= c->nm) return fail(HOS_E_INVALID, "member out of range");
+    if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
+    if (src == dst) return HOS_OK;
+    const size_t st = 2 * c->fs * sizeof(cx<double>);
+    HIP_TRY(hipMemcpyAsync(c->U + dst * 2 * c->fs, c->U + src * 2 * c->fs, st, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->Y + dst * 2 * c->fs, c->Y + src * 2 * c->fs, st, hipMemcpyDeviceToDevice, c->stream));
+    double *scal[] = {c->t, c->dt, c->depth, c->mean0, c->H, c->vol};
+    for (double *a : scal)
+        HIP_TRY(hipMemcpyAsync(a + dst, a + src, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->host_t[dst] = c->host_t[src];
+    c->host_dt[dst] = c->host_dt[src];
+    return HOS_OK;
 }
 
 extern "C" int hos_last_step_ms(hos_ctx *c, double *ms)
@@ -805,12 +911,8 @@ extern "C" int hos_set_tile(hos_ctx *c, int kind, int tx)
     }
     for (int k = HOS_TILE_C2R; k <= HOS_TILE_PAIR; ++k)
         if (kind == HOS_TILE_ALL || kind == k) c->tx[k] = tx ? tx : default_tx(c->f32, c->logny, k);
-    if (c->gexec) {
-        (void)hipGraphExecDestroy(c->gexec);
-        (void)hipGraphDestroy(c->graph);
-        c->gexec = nullptr;
-        c->graph = nullptr;
-    }
+    if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
+    drop_graphs(c);
     return HOS_OK;
 }
 

@@ -28,11 +28,12 @@ constexpr int MAXOPS = 16;
 // wavenumbers: kx from a table (reference or signed), ky = j*Ky0,
 // operators_par.c:50-71, 105-116, 168-192.
 struct KTab {
-    const double *kx;  // [Nx]
+    const double *kx;     // [Nx]
     double ky0;
-    double depth;      // <= 0: deep water
-    int nk;            // Ny/2+1
-    int mx, my;        // dealias bounds, operators_par.c:290-291
+    const double *depth;  // [members]; <= 0: deep water
+    int nk;               // Ny/2+1
+    int mx, my;           // dealias bounds, operators_par.c:290-291
+    __device__ __forceinline__ double h(int m) const { return depth[m]; }
 };
 
 // d^p/dz^p multiplier at |k| = kz: kz^p (deep), kz^p tanh(kz h) for odd p.
@@ -89,7 +90,8 @@ __global__ __launch_bounds__(NT) void k_xinv(const cx<double> *__restrict__ Y, l
     const int ky0 = blockIdx.x * ROWS;
     const cx<double> *src = Y + blockIdx.z * sY + ops.src[grp] * fs;
     cx<double> s[PER];
-    double kxv[PER], kz[PER];
+    double kxv[PER], kz[PER], th[PER];
+    const double h = kt.h(blockIdx.z);
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int e = threadIdx.x + q * NT;
@@ -97,11 +99,13 @@ __global__ __launch_bounds__(NT) void k_xinv(const cx<double> *__restrict__ Y, l
         s[q] = {0.0, 0.0};
         kxv[q] = 0.0;
         kz[q] = 0.0;
+        th[q] = 1.0;
         if (e < ROWS * NX && ky < kt.nk) {
             s[q] = src[(long)ky * NX + kx];
             kxv[q] = kt.kx[kx];
             const double kyv = ky * kt.ky0;
             kz[q] = sqrt(kxv[q] * kxv[q] + kyv * kyv);
+            if (h > 0.0) th[q] = tanh(kz[q] * h);  // d^n/dz^n at finite depth: k^n tanh(kh)^(n odd)
         }
     }
     cx<T> *base = I + blockIdx.z * sI;
@@ -121,7 +125,9 @@ __global__ __launch_bounds__(NT) void k_xinv(const cx<double> *__restrict__ Y, l
                 re = -k * s[q].y;
                 im = k * s[q].x;
             } else if (op.op == OP_DZ) {
-                const double m = dz_mul(kz[q], op.pow, kt.depth);
+                double m = 1.0;
+                for (int i = 0; i < op.pow; ++i) m *= kz[q];
+                if (op.pow & 1) m *= th[q];
                 re = m * s[q].x;
                 im = m * s[q].y;
             }
@@ -402,6 +408,7 @@ __global__ __launch_bounds__(NT) void k_xfwdinv(cx<T> *__restrict__ I, long sI, 
                                                 double scale, KTab kt, const cx<T> *__restrict__ twx)
 {
     constexpr int NX = 1 << LOGNX;
+    constexpr int PER = (ROWS * NX + NT - 1) / NT;
     __shared__ cx<T> S[ROWS * NX];
     __shared__ cx<T> W[ROWS * NX];
     const int ky0 = blockIdx.x * ROWS;
@@ -411,12 +418,28 @@ __global__ __launch_bounds__(NT) void k_xfwdinv(cx<T> *__restrict__ I, long sI, 
         const int r = e / NX, kx = e % NX, ky = ky0 + r;
         S[e] = (ky < kt.nk) ? in[(long)ky * NX + kx] : cx<T>{T(0), T(0)};
     }
+    // |k| and tanh(|k| h) of this thread's modes, FP64
+    double kz[PER], th[PER];
+    const double h = kt.h(blockIdx.z);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = threadIdx.x + q * NT;
+        const int kx = e % NX, ky = ky0 + e / NX;
+        const double kxv = kt.kx[kx], kyv = ky * kt.ky0;
+        kz[q] = sqrt(kxv * kxv + kyv * kyv);
+        th[q] = h > 0.0 ? tanh(kz[q] * h) : 1.0;
+    }
     lds_fft<T, LOGNX, ROWS, NT, false>(S, twx);
+    double kj[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) kj[q] = scale;
     for (int j = 1; j <= ndst; ++j) {
-        for (int e = threadIdx.x; e < ROWS * NX; e += NT) {
-            const int r = e / NX, kx = e % NX, ky = ky0 + r;
-            const double kxv = kt.kx[kx], kyv = ky * kt.ky0;
-            const T mul = T(scale * dz_mul(sqrt(kxv * kxv + kyv * kyv), j, kt.depth));
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = threadIdx.x + q * NT;
+            if (e >= ROWS * NX) break;
+            kj[q] *= kz[q];  // scale * |k|^j
+            const T mul = T((j & 1) ? kj[q] * th[q] : kj[q]);
             const cx<T> s = S[e];
             W[e] = {mul * s.x, mul * s.y};
         }
@@ -496,7 +519,7 @@ __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, lo
         }
         const cx<double> ye = y[idx], yp = y[fsd + idx];
         const double kxv = kt.kx[kx], kyv = ky * kt.ky0;
-        const double w1 = dz_mul(sqrt(kxv * kxv + kyv * kyv), 1, kt.depth);
+        const double w1 = dz_mul(sqrt(kxv * kxv + kyv * kyv), 1, kt.h(m));
         // eta_t = W^(1) + R*(rest), phi_t = -g eta + R*(rest)  (hosm_2d_par.c:211-229)
         cx<double> ke = {w1 * yp.x + R * scale * double(na.x), w1 * yp.y + R * scale * double(na.y)};
         cx<double> kp = {-rk.g * ye.x + R * scale * double(nb.x), -rk.g * ye.y + R * scale * double(nb.y)};
@@ -593,7 +616,7 @@ struct CombArgs {
     double coef[8];
     int np;
 };
-__global__ __launch_bounds__(NT) void k_combine(const cx<double> *__restrict__ U, const cx<double> *__restrict__ K,
+__global__ __launch_bounds__(NT) void k_combine(const cx<double> *U, const cx<double> *__restrict__ K,
                                                 long sK_stage, long sM, cx<double> *out, long n, const double *dt,
                                                 CombArgs ca)
 {
@@ -629,7 +652,7 @@ __global__ __launch_bounds__(NT) void k_spec_op(const cx<double> *__restrict__ a
             r = {-k * s.y, k * s.x};
         } else if (op == SOP_DZ) {
             const double kxv = kt.kx[kx], kyv = ky * kt.ky0;
-            const double m = dz_mul(sqrt(kxv * kxv + kyv * kyv), 1, kt.depth);
+            const double m = dz_mul(sqrt(kxv * kxv + kyv * kyv), 1, kt.h(0));
             r = {m * s.x, m * s.y};
         } else if (op == SOP_SUM) {
             r = {s.x + b[i].x, s.y + b[i].y};

@@ -85,6 +85,108 @@ static void get_params(const char *fn)
     Ky0 = 2 * M_PI / Ly;
 }
 
+/* ASCII form of the same parameters (README.md:35 names an ASCII input that
+ * the reference snapshot does not contain): "key = value" lines, '#'
+ * comments, the initpars.h5 keys plus the optional run settings nlevs,
+ * scheme, dt, depth and fp32.  Returns 0 if the file does not exist. */
+static int get_params_ascii(const char *fn, int *scheme, double *dt)
+{
+    FILE *f = fopen(fn, "r");
+    if (!f) return 0;
+    char line[256];
+    int seen = 0;
+    while (fgets(line, sizeof line, f)) {
+        char *h = strchr(line, '#');
+        if (h) *h = 0;
+        char key[64];
+        double v;
+        if (sscanf(line, " %63[A-Za-z0-9_] = %lf", key, &v) != 2) continue;
+        seen++;
+        if (!strcmp(key, "Nx")) Nx = (int)v;
+        else if (!strcmp(key, "Ny")) Ny = (int)v;
+        else if (!strcmp(key, "Lx")) Lx = v;
+        else if (!strcmp(key, "Ly")) Ly = v;
+        else if (!strcmp(key, "g")) g = v;
+        else if (!strcmp(key, "runsubid")) runsubid = (int)v;
+        else if (!strcmp(key, "T")) T = v;
+        else if (!strcmp(key, "dtsave")) dtsave = v;
+        else if (!strcmp(key, "saveflg")) saveflg = (int)v;
+        else if (!strcmp(key, "rampflg")) rampflg = (int)v;
+        else if (!strcmp(key, "Tramp")) Tramp = v;
+        else if (!strcmp(key, "nlevs")) hos_nlevs = (int)v;
+        else if (!strcmp(key, "scheme")) *scheme = (int)v;
+        else if (!strcmp(key, "dt")) *dt = v;
+        else if (!strcmp(key, "depth")) hos_depth = v;
+        else if (!strcmp(key, "fp32")) hos_precision = v != 0 ? HOS_FP32 : HOS_FP64;
+        else {
+            fprintf(stderr, "%s: unknown key %s\n", fn, key);
+            exit(EXIT_FAILURE);
+        }
+    }
+    fclose(f);
+    if (!(Nx > 0 && Ny > 0 && Lx > 0 && Ly > 0 && g > 0 && dtsave > 0)) {
+        fprintf(stderr, "%s: Nx, Ny, Lx, Ly, g, dtsave are required\n", fn);
+        exit(EXIT_FAILURE);
+    }
+    if (rampflg != 0 && rampflg != 1) rampflg = 0;
+    Kx0 = 2 * M_PI / Lx;
+    Ky0 = 2 * M_PI / Ly;
+    return seen;
+}
+
+/* XDMF index for ParaView (xdmf_routines_2d_par.c:26-146): one temporal
+ * collection of Polyvertex grids over data_coordinates.h5:/xy.  Unlike the
+ * reference, /xy holds the physical coordinates (x, y) of node i*Ny + j (the
+ * reference writes wavenumbers there), the data type is spelled out, and the
+ * coordinates are built on the heap rather than in a stack array. */
+static void xml_grid(FILE *x, double t, const char *datafile)
+{
+    const long n = (long)Nx * Ny;
+    fprintf(x, "<Grid Name=\"Cells\" GridType=\"Uniform\">\n<Time Value=\"%f\"/>\n", t);
+    fprintf(x, "<Topology TopologyType=\"Polyvertex\" NodesPerElement=\"%ld\">\n</Topology>\n", n);
+    fprintf(x, "<Geometry GeometryType=\"XY\">\n<DataItem DataType=\"Float\" Precision=\"8\" Dimensions=\"%ld 2\" "
+               "Format=\"HDF\">data_coordinates.h5:/xy</DataItem>\n</Geometry>\n", n);
+    const char *names[3][2] = {{"Phi", "phi"}, {"Eta", "eta"}, {"Bathy", "bat"}};
+    for (int q = 0; q < 3; q++)
+        fprintf(x, "<Attribute AttributeType=\"Scalar\" Center=\"Node\" Name=\"%s\">\n<DataItem DataType=\"Float\" "
+                   "Precision=\"8\" Dimensions=\"%ld 1\" Format=\"HDF\">%s:/%s</DataItem>\n</Attribute>\n",
+                names[q][0], n, datafile, names[q][1]);
+    fprintf(x, "</Grid>\n");
+}
+
+static FILE *init_xml(double t, const char *datafile)
+{
+    const long n = (long)Nx * Ny;
+    double *xy = malloc(sizeof(double) * 2 * n);
+    if (!xy) return NULL;
+    for (int i = 0; i < Nx; i++)
+        for (int j = 0; j < Ny; j++) {
+            xy[2 * ((long)i * Ny + j)] = i * Lx / Nx;
+            xy[2 * ((long)i * Ny + j) + 1] = j * Ly / Ny;
+        }
+    hid_t f = H5Fcreate("data_coordinates.h5", H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+    hsize_t dim[2] = {(hsize_t)n, 2};
+    hid_t sp = H5Screate_simple(2, dim, NULL);
+    hid_t d = H5Dcreate2(f, "xy", H5T_IEEE_F64LE, sp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    H5Dwrite(d, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, xy);
+    H5Dclose(d);
+    H5Sclose(sp);
+    H5Fclose(f);
+    free(xy);
+    FILE *x = fopen("HOS_data.xmf", "w");
+    if (!x) return NULL;
+    fprintf(x, "<Xdmf>\n<Domain>\n<Grid Name=\"CellTime\" GridType=\"Collection\" CollectionType=\"Temporal\">\n");
+    xml_grid(x, t, datafile);
+    return x;
+}
+
+static void close_xml(FILE *x)
+{
+    if (!x) return;
+    fprintf(x, "</Grid>\n</Domain>\n</Xdmf>\n");
+    fclose(x);
+}
+
 /* get_ic_2d, hdf5_routines_2d_par.c:104-191: [Nx][Ny] -> padded [Nx][Ny+2] */
 static void read_field(hid_t f, const char *name, double *tmp, double *out)
 {
@@ -181,7 +283,12 @@ int main(int argc, char **argv)
     struct timespec w0, w1;
     clock_gettime(CLOCK_MONOTONIC, &w0);
 
-    get_params("initpars.h5");
+    if (access("initpars.h5", R_OK) == 0)
+        get_params("initpars.h5");
+    else if (!get_params_ascii("initpars.txt", &scheme, &dt)) {
+        fprintf(stderr, "neither initpars.h5 nor initpars.txt found\n");
+        return 1;
+    }
     char subid[2], name[128];
     snprintf(subid, 2, "%d", runsubid); /* one digit, main_2d_par.c:93 */
     snprintf(name, sizeof name, "initdata.%s.h5", subid);
@@ -248,6 +355,7 @@ int main(int argc, char **argv)
     field(s, "phi", phi, tmp);
     field(s, "bat", bat, tmp);
     H5Fclose(s);
+    FILE *xml = init_xml(t, name); /* main_2d_par.c:276-277 */
     printf("Datafile '%s' written at t=%f\n", name, t);
     if (saveflg > 1) {
         snprintf(name, sizeof name, "data_extra%d.%s.h5", nfld, subid);
@@ -273,6 +381,7 @@ int main(int argc, char **argv)
             field(s, "phi", phi, tmp);
             field(s, "bat", bat, tmp);
             H5Fclose(s);
+            if (xml) xml_grid(xml, t, name); /* write_xml, main_2d_par.c:328 */
             printf("Datafile '%s' written at t=%f\n", name, t);
             if (saveflg > 1) {
                 snprintf(name, sizeof name, "data_extra%d.%s.h5", nfld, subid);
@@ -284,6 +393,7 @@ int main(int argc, char **argv)
             }
         }
     }
+    close_xml(xml);
     hos_ref_sync();
     rhs_hos_clean();
     clock_gettime(CLOCK_MONOTONIC, &w1);

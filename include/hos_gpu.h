@@ -98,6 +98,25 @@ int hos_step(hos_ctx *c, int nsteps, double dt);
 /* Same, with a per-member dt array (length nmembers). */
 int hos_step_dt(hos_ctx *c, int nsteps, const double *dt);
 
+/* Batching and slot pools (ensembles, the serving loop):
+ *  hos_set_active(c, n)   only members 0..n-1 are stepped (0 <= n <= nmembers);
+ *  hos_get_active(c)      the current count;
+ *  hos_set_member(c, m, depth, dt, t)   per-member depth (<= 0: deep), time
+ *                         step and time;
+ *  hos_copy_member(c, src, dst)   copy a member's state and parameters to
+ *                         another slot (slot compaction);
+ *  hos_step_async(c, n)   enqueue n steps of the active members with the
+ *                         per-member dt already set, and return at once;
+ *  hos_wait(c)            wait for them and record the step timings.
+ * Contexts use their own HIP stream, so the steps of several contexts
+ * enqueued with hos_step_async run concurrently on the device. */
+int hos_set_active(hos_ctx *c, int n);
+int hos_get_active(hos_ctx *c);
+int hos_set_member(hos_ctx *c, int member, double depth, double dt, double t);
+int hos_copy_member(hos_ctx *c, int src, int dst);
+int hos_step_async(hos_ctx *c, int nsteps);
+int hos_wait(hos_ctx *c);
+
 /* Stage-0 Hamiltonian of the last step, hosm_2d_par.c:462-523 (the
  * reference's H = mean(g eta^2 + phi eta_t), no 1/2), and volume = mean eta =
  * Re eta-hat(0,0), both FP64. */

@@ -59,3 +59,35 @@ def test_driver_matches_oracle(tmp_path, oracle_mod, scheme, nlevs):
     assert rt.shape == (nsteps, 2)
     assert np.allclose(rt[:, 0], np.arange(nsteps) * dt, atol=1e-9)
     assert np.allclose(rt[:, 1], H, rtol=1e-12)
+    # XDMF index: one grid per snapshot over physical coordinates
+    xmf = open(os.path.join(d, "HOS_data.xmf")).read()
+    assert xmf.count("<Grid Name=\"Cells\"") == 1 + nsteps // 4 and xmf.rstrip().endswith("</Xdmf>")
+    xy = read(os.path.join(d, "data_coordinates.h5"), "/xy", (Nx * Ny, 2))
+    assert xy[Ny + 1, 0] == pytest.approx(L / Nx) and xy[1, 1] == pytest.approx(L / Ny)
+
+
+@pytest.mark.skipif(not have_h5dump(), reason="h5dump not in this image")
+def test_driver_ascii_parameters(tmp_path):
+    """initpars.txt (key = value) in place of initpars.h5 gives the same run."""
+    print("driver ascii", flush=True)
+    Nx = Ny = 32
+    L = 2 * 2 * math.pi
+    dt = 0.05
+    runs = []
+    for form in ("h5", "txt"):
+        d = tmp_path / form
+        d.mkdir()
+        subprocess.run([os.path.join(BIN, "hos_mkic"), str(Nx), str(Ny), repr(L), repr(L), "9.81", "0", "1", "3.3",
+                        "0.1", "4", "100", "0.2", "1", str(d)], check=True, timeout=120)
+        args = ["-n", "2", "-s", "2", "-t", repr(dt)]
+        if form == "txt":
+            os.remove(d / "initpars.h5")
+            (d / "initpars.txt").write_text(
+                f"# same run as initpars.h5\nNx = {Nx}\nNy = {Ny}\nLx = {L!r}\nLy = {L!r}\ng = 9.81\n"
+                "runsubid = 1\nT = 100\ndtsave = 0.2\nsaveflg = 1\nrampflg = 0\nTramp = 1\n"
+                f"nlevs = 2\nscheme = 2\ndt = {dt!r}\n")
+            args = []
+        subprocess.run([os.path.join(BIN, "hos_driver"), "-d", str(d), *args, "-N", "8"], check=True,
+                       capture_output=True, timeout=120)
+        runs.append(read(str(d / "data2.1.h5"), "/eta"))
+    assert np.array_equal(runs[0], runs[1])

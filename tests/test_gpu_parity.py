@@ -130,6 +130,44 @@ def test_members_match_alone(hos):
         assert np.array_equal(alone.get_state(), batch.get_state(i))
 
 
+def test_slots_active_count_and_member_params(hos):
+    """Per-member depth/dt, stepping a prefix of the slots, slot copies, and
+    two contexts stepped concurrently (async): each member matches its run
+    alone bit for bit."""
+    Nx, Ny, nlevs = 64, 32, 2
+    print("slots", flush=True)
+    p = hos.params(Nx=Nx, Ny=Ny, nlevs=nlevs, precision=1, Lx=8.0, Ly=4.0)
+    depths, dts = [0.0, 0.6, 1.4, 0.0], [0.02, 0.025, 0.03, 0.015]
+    states = [random_state(Nx, Ny, nlevs, amp=0.05, seed=40 + i) for i in range(4)]
+    pool = hos.Solver(p, 4)
+    other = hos.Solver(p, 1)
+    for i in range(4):
+        pool.set_state(states[i], i)
+        pool.set_member(i, depths[i], dts[i])
+    other.set_state(states[3])
+    other.set_member(0, depths[3], dts[3])
+    pool.use_graph(True)
+    pool.set_active(2)
+    pool.step_async(3)
+    other.step_async(3)
+    pool.wait()
+    other.wait()
+    assert np.array_equal(pool.get_state(2), states[2])  # not active: untouched
+    for i in range(2):
+        alone = hos.Solver(hos.params(Nx=Nx, Ny=Ny, nlevs=nlevs, precision=1, Lx=8.0, Ly=4.0, depth=depths[i]))
+        alone.set_state(states[i])
+        alone.step(3, dts[i])
+        assert np.array_equal(alone.get_state(), pool.get_state(i))
+        assert alone.get_time() == pytest.approx(pool.get_time(i))
+    pool.copy_member(1, 3)
+    ref1 = pool.get_state(1)
+    assert np.array_equal(pool.get_state(3), ref1)
+    pool.set_active(4)
+    pool.step_async(2)
+    pool.wait()
+    assert np.array_equal(pool.get_state(3), pool.get_state(1))
+
+
 def test_graph_replay_matches_eager(hos):
     Nx, Ny, nlevs = 64, 64, 4
     print("graph", flush=True)
