@@ -172,6 +172,99 @@ def jonswap(Nx, Ny, Lx, Ly, kp=1.0, gamma=3.3, eps=0.1, depth=0.0, g=9.81, theta
     return u
 
 
+class ServerParams(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_double), ("precision", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("max_slots", ctypes.c_int), ("compensated", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("Nx", ctypes.c_int), ("Ny", ctypes.c_int), ("nlevs", ctypes.c_int), ("Lx", ctypes.c_double),
+                ("Ly", ctypes.c_double), ("g", ctypes.c_double), ("depth", ctypes.c_double),
+                ("dt", ctypes.c_double), ("nsteps", ctypes.c_int)]
+
+
+SERVE_EXPORTS = ["hos_server_create", "hos_server_destroy", "hos_server_submit", "hos_server_status",
+                 "hos_server_result", "hos_server_step", "hos_server_start", "hos_server_stop",
+                 "hos_server_drain", "hos_server_stats"]
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+
+
+class Server:
+    """hos_server (include/hos_serve.h): continuous batching of many runs."""
+
+    def __init__(self, hbm_budget=250e9, precision=HOS_FP32, use_graph=True, max_slots=0, compensated=True):
+        lib = load()
+        vp = ctypes.c_void_p
+        lib.hos_server_create.argtypes = [ctypes.POINTER(ServerParams), ctypes.POINTER(vp)]
+        lib.hos_server_destroy.argtypes = [vp]
+        lib.hos_server_submit.argtypes = [vp, ctypes.POINTER(Request), vp, ctypes.POINTER(ctypes.c_longlong)]
+        lib.hos_server_status.argtypes = [vp, ctypes.c_longlong]
+        lib.hos_server_result.argtypes = [vp, ctypes.c_longlong, vp, vp]
+        lib.hos_server_step.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+        lib.hos_server_start.argtypes = [vp]
+        lib.hos_server_stop.argtypes = [vp]
+        lib.hos_server_drain.argtypes = [vp, ctypes.c_double]
+        lib.hos_server_stats.argtypes = [vp, vp, ctypes.c_int]
+        self.lib = lib
+        sp = ServerParams(float(hbm_budget), precision, int(use_graph), max_slots, int(compensated))
+        h = vp()
+        if lib.hos_server_create(ctypes.byref(sp), ctypes.byref(h)) != 0:
+            raise HosError("hos_server_create rejected its parameters")
+        self.h = h
+        self.shapes = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.hos_server_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def submit(self, u0, Nx, Ny, nlevs, Lx, Ly, dt, nsteps, depth=0.0, g=9.81) -> int:
+        u0 = np.ascontiguousarray(u0, dtype=np.complex128)
+        assert u0.shape == (2, Ny // 2 + 1, Nx)
+        r = Request(Nx, Ny, nlevs, Lx, Ly, g, depth, dt, nsteps)
+        rid = ctypes.c_longlong()
+        rc = self.lib.hos_server_submit(self.h, ctypes.byref(r), _ptr(u0), ctypes.byref(rid))
+        if rc != 0:
+            raise HosError(f"submit rejected ({rc}): {load().hos_last_error().decode()}")
+        self.shapes[rid.value] = (Nx, Ny)
+        return rid.value
+
+    def status(self, rid: int) -> int:
+        return self.lib.hos_server_status(self.h, rid)
+
+    def result(self, rid: int):
+        Nx, Ny = self.shapes.pop(rid)
+        u = np.zeros((2, Ny // 2 + 1, Nx), np.complex128)
+        info = np.zeros(6)
+        rc = self.lib.hos_server_result(self.h, rid, _ptr(u), _ptr(info))
+        if rc != 0:
+            raise HosError(f"request {rid} failed ({rc})")
+        return u, dict(submit=info[0], admit=info[1], done=info[2], steps=int(info[3]), H=info[4], vol=info[5])
+
+    def step(self) -> int:
+        n = ctypes.c_int()
+        _check(self.lib.hos_server_step(self.h, ctypes.byref(n)))
+        return n.value
+
+    def start(self):
+        _check(self.lib.hos_server_start(self.h))
+
+    def stop(self):
+        _check(self.lib.hos_server_stop(self.h))
+
+    def drain(self, timeout=600.0):
+        _check(self.lib.hos_server_drain(self.h, timeout))
+
+    def stats(self) -> dict:
+        v = np.zeros(12)
+        _check(self.lib.hos_server_stats(self.h, _ptr(v), 12))
+        keys = ["step_p50_ms", "step_p99_ms", "req_p50_s", "req_p99_s", "completed", "pt_steps", "busy_s",
+                "pools", "peak_bytes", "peak_running", "wait_p50_s", "wait_p99_s"]
+        return dict(zip(keys, v.tolist()))
+
+
 def peak_period(g, kp, depth=0.0) -> float:
     return load().hos_ic_peak_period(g, kp, depth)
 

@@ -76,12 +76,15 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, const cx<T> *__restrict__ 
 #pragma unroll
             for (int r = 0; r < R; ++r) v[q][r] = src[j + r * NR];
             if constexpr (NS > 1) {
+                // w^r for r = 1..R-1 from one table entry w = exp(-+2 pi i k/(NS R))
                 const int k = j % NS;
+                cx<T> w = tw[k * (N / (NS * R))];
+                if (INV) w.y = -w.y;
+                cx<T> wr = w;
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    cx<T> w = tw[k * r * (N / (NS * R))];
-                    if (INV) w.y = -w.y;
-                    v[q][r] = cmul(v[q][r], w);
+                    v[q][r] = cmul(v[q][r], wr);
+                    if (r + 1 < R) wr = cmul(wr, w);
                 }
             }
             dft_small<T, R, INV>(v[q]);

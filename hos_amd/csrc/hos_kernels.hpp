@@ -349,9 +349,16 @@ __global__ __launch_bounds__(NT) void k_yr2c(PRO pro, cx<T> *__restrict__ I, lon
     __shared__ cx<T> buf[NPR * NY];
     const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TX;
     const int m = blockIdx.z;
-    for (int e = threadIdx.x; e < NPR * NY; e += NT) {
-        const int p = e / NY, y = e % NY;
-        buf[e] = {pro(m, x0 + 2 * p, y), pro(m, x0 + 2 * p + 1, y)};
+    // compile-time trip count, unrolled: the loads of several points are in
+    // flight together (the prologue reads up to M+1 fields per point)
+    constexpr int PER = (NPR * NY + NT - 1) / NT;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = threadIdx.x + q * NT;
+        if (e < NPR * NY) {
+            const int p = e / NY, y = e % NY;
+            buf[e] = {pro(m, x0 + 2 * p, y), pro(m, x0 + 2 * p + 1, y)};
+        }
     }
     lds_fft<T, LOGNY, NPR, NT, false>(buf, twy);
     cx<T> *out = I + m * sI + dst * fsI;
@@ -378,11 +385,16 @@ __global__ __launch_bounds__(NT) void k_yr2c_pair(PRO pro, cx<T> *__restrict__ I
     __shared__ cx<T> buf[TXP * NY];
     const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TXP;
     const int m = blockIdx.z;
-    for (int e = threadIdx.x; e < TXP * NY; e += NT) {
-        const int p = e / NY, y = e % NY;
-        T a, b;
-        pro(m, x0 + p, y, a, b);
-        buf[e] = {a, b};
+    constexpr int PER = (TXP * NY + NT - 1) / NT;
+#pragma unroll 2
+    for (int q = 0; q < PER; ++q) {
+        const int e = threadIdx.x + q * NT;
+        if (e < TXP * NY) {
+            const int p = e / NY, y = e % NY;
+            T a, b;
+            pro(m, x0 + p, y, a, b);
+            buf[e] = {a, b};
+        }
     }
     lds_fft<T, LOGNY, TXP, NT, false>(buf, twy);
     cx<T> *oa = I + m * sI + dstA * fsI;

@@ -24,6 +24,24 @@ def test_library_exports_every_declared_symbol(hos):
     ic = declared("hos_ic.h")
     assert sorted(ic) == sorted(hos.IC_EXPORTS)
     assert all(hasattr(lib, n) for n in ic)
+    srv = declared("hos_serve.h")
+    assert sorted(srv) == sorted(hos.SERVE_EXPORTS)
+    assert all(hasattr(lib, n) for n in srv)
+
+
+def test_reference_boundary_exports():
+    """libhos_ref.so exports the reference's own entry points (hos_ref.h)."""
+    lib = ctypes.CDLL(os.path.join(ROOT, "hos_amd", "libhos_ref.so"))
+    names = ["Setup_TimeScheme", "sol_update_RK", "rhs_hos_setup", "rhs_hos_clean", "rhs_hos", "Zvel",
+             "ZvelLinear", "Hamiltonian", "RampFun", "Dx", "Dy", "Dz", "Mult", "Sum", "Dealias", "fft_2d",
+             "ifft_2d", "hos_local_size_2d_transposed", "hos_ref_sync", "hos_ref_host_modified",
+             "hos_ref_context"]
+    txt = open(os.path.join(ROOT, "include", "hos_ref.h")).read()
+    for n in names:
+        assert hasattr(lib, n), n
+        assert re.search(r"\b%s\s*\(" % n, txt), n
+    for g in ("Nx", "Ny", "Lx", "Ly", "g", "alloc_local", "local_N", "hos_nlevs", "hos_precision"):
+        assert hasattr(lib, g), g
 
 
 def test_params_default_and_footprint(hos):

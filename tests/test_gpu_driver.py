@@ -4,6 +4,7 @@ over libhos_ref.so) runs it and writes reference-format snapshots, which must
 match the oracle run from the same input."""
 import math
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -35,7 +36,7 @@ def test_driver_matches_oracle(tmp_path, oracle_mod, scheme, nlevs):
     out = subprocess.run([os.path.join(BIN, "hos_driver"), "-d", d, "-n", str(nlevs), "-s", str(scheme), "-t",
                           repr(dt), "-N", str(nsteps)], check=True, capture_output=True, text=True, timeout=120)
     print(out.stdout[-400:], flush=True)
-    files = sorted(f for f in os.listdir(d) if f.startswith("data") and not f.startswith("data_extra"))
+    files = sorted(f for f in os.listdir(d) if re.fullmatch(r"data\d+\.1\.h5", f))
     assert len(files) == 1 + nsteps // 4  # data0 + one per dtsave crossing
     last = os.path.join(d, f"data{nsteps // 4}.1.h5")
     assert read(last, "/time")[0] == pytest.approx(nsteps * dt, rel=1e-12)
