@@ -44,13 +44,14 @@ def make_state(hos, w):
                        depth=w["depth"], g=w["g"], seed=w["seed"], nlevs=w["nlevs"])
 
 
-def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_steps=0):
+def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_steps=0, autotune=True):
     p = hos.params(Nx=w["Nx"], Ny=w["Ny"], nlevs=w["nlevs"], Lx=w["Lx"], Ly=w["Ly"], g=w["g"],
                    depth=w["depth"], precision=prec, compensated=1)
     s = hos.Solver(p, members)
     for m in range(members):
         s.set_state(u0, m)
-    H0, v0 = None, None
+        s.set_member(m, w["depth"], dt)
+    tiles = s.autotune(3) if autotune else None
     s.use_graph(graph)
     s.step(warmup, dt)
     s.sync()
@@ -69,6 +70,7 @@ def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_s
         "vol": s.energy(0)[1],
         "state": s.get_state(0),
         "bytes_per_pt_step": hos.bytes_per_point_step(p),
+        "tiles": tiles,
     }
     if profile_steps:
         # same workload, stream-ordered launches with a HIP event pair around
@@ -176,7 +178,7 @@ def main():
         "config": {
             "workload": "1024x1024 grid, 32x32 peak wavelengths, kp*h=1, M=5, JONSWAP gamma=5 kp*Hs/2=0.12 "
                         "seed 1, RK4 dt=Tp/100",
-            "members": args.members, "graph": not args.no_graph,
+            "members": args.members, "graph": not args.no_graph, "y_tiles_c2r_r2c_pair": f32["tiles"],
             "form": "FP32 FFTs/products with Kahan order sums; FP64 state, RK combination, linear terms",
             "parallelism": "single GPU",
         },

@@ -43,6 +43,7 @@ EXPORTS = [
     "hos_last_error", "hos_device_name", "hos_num_members", "hos_bytes_per_point_step", "hos_step_times",
     "hos_profile", "hos_kernel_stats", "hos_spectral_op", "hos_mult", "hos_hamiltonian",
     "hos_set_active", "hos_get_active", "hos_set_member", "hos_copy_member", "hos_step_async", "hos_wait",
+    "hos_autotune",
 ]
 HOS_OP_DX, HOS_OP_DY, HOS_OP_DZ, HOS_OP_SUM, HOS_OP_DEALIAS = 1, 2, 3, 4, 5
 # include/hos_ic.h
@@ -107,6 +108,7 @@ def load():
     lib.hos_spectral_op.argtypes = [vp, ip, vp, vp, vp]
     lib.hos_mult.argtypes = [vp, vp, vp, vp]
     lib.hos_hamiltonian.argtypes = [vp, vp, vp, vp, dp]
+    lib.hos_autotune.argtypes = [vp, ip, vp]
     lib.hos_set_active.argtypes = [vp, ip]
     lib.hos_get_active.argtypes = [vp]
     lib.hos_set_member.argtypes = [vp, ip, ctypes.c_double, ctypes.c_double, ctypes.c_double]
@@ -407,6 +409,12 @@ class Solver:
 
     def use_graph(self, on: bool = True):
         _check(self.lib.hos_use_graph(self.h, int(on)))
+
+    def autotune(self, nsteps: int = 2):
+        """hos_autotune: returns the chosen (c2r, r2c, pair) y tiles."""
+        t = (ctypes.c_int * 3)()
+        _check(self.lib.hos_autotune(self.h, nsteps, t))
+        return tuple(t)
 
     def set_tile(self, tx: int, kind: int = 3):
         """y-pass tile (4, 8, 16; 0 = default) for kind HOS_TILE_C2R (0),

@@ -20,7 +20,11 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <array>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/hos_gpu.h"
@@ -913,6 +917,85 @@ extern "C" int hos_set_tile(hos_ctx *c, int kind, int tx)
         if (kind == HOS_TILE_ALL || kind == k) c->tx[k] = tx ? tx : default_tx(c->f32, c->logny, k);
     if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
     drop_graphs(c);
+    return HOS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: y tiles per kernel kind, timed on the context's own members
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, int, int, int, int, int>, std::array<int, 3>> g_tune_cache;
+
+extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
+{
+    if (!c) return fail(HOS_E_INVALID, "null context");
+    if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
+    if (nsteps < 1) nsteps = 2;
+    const auto key = std::make_tuple(c->p.Nx, c->p.Ny, c->M, (int)c->f32, c->active, (int)c->subdiag);
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            for (int k = 0; k < 3; ++k) c->tx[k] = it->second[k];
+            drop_graphs(c);
+            if (tiles_out)
+                for (int k = 0; k < 3; ++k) tiles_out[k] = c->tx[k];
+            return HOS_OK;
+        }
+    }
+    // save what a trial step changes: U, Y, t, H, volume
+    const size_t st = (size_t)c->nm * 2 * c->fs * sizeof(cx<double>);
+    void *saveU = nullptr, *saveY = nullptr;
+    HIP_TRY(hipMalloc(&saveU, st));
+    if (hipMalloc(&saveY, st) != hipSuccess) {
+        (void)hipFree(saveU);
+        return fail(HOS_E_NOMEM, "autotune scratch");
+    }
+    std::vector<double> t_save = c->host_t;
+    (void)hipMemcpyAsync(saveU, c->U, st, hipMemcpyDeviceToDevice, c->stream);
+    (void)hipMemcpyAsync(saveY, c->Y, st, hipMemcpyDeviceToDevice, c->stream);
+    std::vector<double> dtev(c->nm);
+    for (int m = 0; m < c->nm; ++m) dtev[m] = c->host_dt[m] > 0 ? c->host_dt[m] : 1e-3;
+    int rc = upload_dt(c, dtev.data());
+    const int cand[3] = {4, 8, 16};
+    std::array<int, 3> best = {c->tx[0], c->tx[1], c->tx[2]};
+    double best_ms = 1e30;
+    const int saved_graph = c->use_graph;
+    c->use_graph = 1;
+    for (int a = 0; a < 3 && !rc; ++a)
+        for (int b = 0; b < 3 && !rc; ++b)
+            for (int d = 0; d < 3 && !rc; ++d) {
+                const int tt[3] = {cand[a], cand[b], cand[d]};
+                bool ok = true;
+                for (int k = 0; k < 3; ++k) ok &= tx_fits(c->f32, c->logny, tt[k]) && c->p.Nx % tt[k] == 0;
+                if (!ok) continue;
+                for (int k = 0; k < 3; ++k) c->tx[k] = tt[k];
+                drop_graphs(c);
+                rc = step_enqueue(c, 1);  // capture + warm
+                if (!rc) rc = step_finish(c);
+                if (!rc) rc = step_enqueue(c, nsteps);
+                if (!rc) rc = step_finish(c);
+                if (!rc && c->last_ms < best_ms) {
+                    best_ms = c->last_ms;
+                    best = {tt[0], tt[1], tt[2]};
+                }
+            }
+    c->use_graph = saved_graph;
+    for (int k = 0; k < 3; ++k) c->tx[k] = best[k];
+    drop_graphs(c);
+    (void)hipMemcpyAsync(c->U, saveU, st, hipMemcpyDeviceToDevice, c->stream);
+    (void)hipMemcpyAsync(c->Y, saveY, st, hipMemcpyDeviceToDevice, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(saveU);
+    (void)hipFree(saveY);
+    c->host_t = t_save;
+    (void)hipMemcpy(c->t, c->host_t.data(), sizeof(double) * c->nm, hipMemcpyHostToDevice);
+    if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_tune_cache[key] = best;
+    }
+    if (tiles_out)
+        for (int k = 0; k < 3; ++k) tiles_out[k] = best[k];
     return HOS_OK;
 }
 

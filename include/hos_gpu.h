@@ -199,6 +199,14 @@ int hos_use_graph(hos_ctx *c, int enable);
 #define HOS_TILE_ALL 3
 int hos_set_tile(hos_ctx *c, int kind, int tx);
 
+/* Autotuner: time nsteps RK steps of the active members for every valid
+ * (c2r, r2c, pair) tile combination, keep the fastest and store it in an
+ * in-process cache keyed by (Nx, Ny, M, precision, active members, scheme
+ * kind); later calls with the same key apply the cached choice at once.
+ * The members' state and time are restored afterwards.  tiles_out[3]
+ * (may be NULL) receives the choice. */
+int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out);
+
 const char *hos_last_error(void);
 
 /* Introspection used by tests and the benchmark. */

@@ -168,6 +168,31 @@ def test_slots_active_count_and_member_params(hos):
     assert np.array_equal(pool.get_state(3), pool.get_state(1))
 
 
+def test_autotune_restores_state_and_tiles_agree(hos):
+    """The autotuner leaves state and time as they were, and every y tile
+    gives the same result (tiles change launch shape, not arithmetic)."""
+    Nx, Ny, nlevs = 64, 64, 3
+    print("autotune", flush=True)
+    p = hos.params(Nx=Nx, Ny=Ny, nlevs=nlevs, precision=1)
+    u = random_state(Nx, Ny, nlevs, amp=0.06, seed=31)
+    s = hos.Solver(p, 2)
+    s.set_state(u, 0)
+    s.set_state(u, 1)
+    s.set_member(0, 0.0, 0.02)
+    s.set_member(1, 0.0, 0.02)
+    tiles = s.autotune(2)
+    assert all(t in (4, 8, 16) for t in tiles)
+    assert np.array_equal(s.get_state(0), u) and s.get_time(0) == 0.0
+    outs = []
+    for tx in (4, 8, 16):
+        r = hos.Solver(p)
+        r.set_state(u)
+        r.set_tile(tx)
+        r.step(3, 0.02)
+        outs.append(r.get_state())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
 def test_graph_replay_matches_eager(hos):
     Nx, Ny, nlevs = 64, 64, 4
     print("graph", flush=True)
