@@ -49,7 +49,7 @@ HOS_OP_DX, HOS_OP_DY, HOS_OP_DZ, HOS_OP_SUM, HOS_OP_DEALIAS = 1, 2, 3, 4, 5
 # include/hos_ic.h
 IC_EXPORTS = ["hos_ic_jonswap", "hos_ic_uniform", "hos_ic_peak_period", "hos_dealias_state"]
 
-KERNEL_KINDS = ["xinv", "yc2r", "yr2c", "yr2c_pair", "xfwdinv", "xfwd_rhs"]
+KERNEL_KINDS = ["xinv", "yc2r", "yr2c", "yr2c_pair", "xfwdinv", "xfwd_rhs", "ylevel", "ylevel_pair"]
 
 
 class Jonswap(ctypes.Structure):

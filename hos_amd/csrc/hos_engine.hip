@@ -491,11 +491,28 @@ struct Eng {
     Phys<T> phys() const { return Phys<T>{P(), sP(), c->fsP, c->p.Ny}; }
     bool comp() const { return c->f32 && c->p.compensated; }
 
-    // ops.n[0] > 0 outputs from source ops.src[0], ops.n[1] >= 0 from ops.src[1]
-    int xinv(const cx<double> *Y, const XinvOps &ops)
+    // outputs list[i] (dst = intermediate index), grouped by source into
+    // chunks of at most `per` outputs, one chunk per blockIdx.y
+    int xinv(const cx<double> *Y, const std::vector<XinvOp> &list, int per = 3)
     {
-        const int ngroups = ops.n[1] > 0 ? 2 : 1;
-        const int nops = ops.n[0] + ops.n[1];
+        XinvOps ops{};
+        for (int s = 0; s < 2; ++s) {
+            int k = 0;
+            for (const XinvOp &o : list) {
+                if (o.src != s) continue;
+                if (k == 0) {
+                    if (ops.ngroups == MAXGRP) return fail(HOS_E_INVALID, "too many x-inverse outputs");
+                    ops.src[ops.ngroups] = s;
+                    ops.n[ops.ngroups] = 0;
+                    ops.ngroups++;
+                }
+                const int g = ops.ngroups - 1;
+                ops.o[g][ops.n[g]++] = o;
+                k = (k + 1) % std::min(per, MAXPERGRP);
+            }
+        }
+        const int ngroups = ops.ngroups;
+        const int nops = (int)list.size();
         LOGN_SWITCH(c->lognx, {
             constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
             dim3 grid((c->Nk + ROWS - 1) / ROWS, ngroups, nm);
@@ -580,18 +597,14 @@ struct Eng {
         return HOS_OK;
     }
 
-    // physical fields of the W expansion from the stage input Y
-    int w_levels(const cx<double> *Y)
+    // level 0: eta, eta_x, eta_y, phi_x, phi_y, d^j phi/dz^j (j = 1..M), the
+    // outputs of the first x inverse pass; returns their count
+    int level0(std::vector<XinvOp> &ops, FieldMap &fm)
     {
         const HosIdx &ix = c->ix;
-        const int M = c->M;
-        XinvOps ops{};
-        FieldMap fm{};
         int n = 0;
-        ops.src[0] = 0;
-        ops.src[1] = 1;
         auto add = [&](int src, int op, int pw, int pdst) {
-            ops.o[src][ops.n[src]++] = {src, op, pw, n};
+            ops.push_back({src, op, pw, n});
             fm.src[n] = n;
             fm.dst[n] = pdst;
             ++n;
@@ -601,7 +614,18 @@ struct Eng {
         add(0, OP_DY, 0, ix.etay);
         add(1, OP_DX, 0, ix.phix);
         add(1, OP_DY, 0, ix.phiy);
-        for (int j = 1; j <= M; ++j) add(1, OP_DZ, j, ix.d[1][j]);
+        for (int j = 1; j <= c->M; ++j) add(1, OP_DZ, j, ix.d[1][j]);
+        return n;
+    }
+
+    // physical fields of the W expansion from the stage input Y
+    int w_levels(const cx<double> *Y)
+    {
+        const HosIdx &ix = c->ix;
+        const int M = c->M;
+        std::vector<XinvOp> ops;
+        FieldMap fm{};
+        const int n = level0(ops, fm);
         int rc = xinv(Y, ops);
         if (!rc) rc = yc2r(fm, n);
         for (int lev = 1; lev <= M - 1 && !rc; ++lev) {
@@ -619,12 +643,52 @@ struct Eng {
         return rc;
     }
 
+    // c2r of nf fields (fm) then the next prologue's r2c, on the same x-tiles
+    template <class PRO, bool PAIR>
+    int ylevel(const FieldMap &fm, int nf, const PRO &pro, int da, int db, int nin)
+    {
+        const int tx = c->tx[PAIR ? HOS_TILE_PAIR : HOS_TILE_R2C];
+        const double bytes = (double)nm * sizeof(T) *
+                             (nf * (2.0 * c->fs + c->fsP) + nin * (double)c->fsP + (PAIR ? 4.0 : 2.0) * c->fs);
+        LOGN_SWITCH(c->logny, {
+            TX_SWITCH(T, tx, {
+                dim3 grid(c->p.Nx / TXV, 1, nm);
+                Prof pf(c, PAIR ? HOS_K_YLEVEL_PAIR : HOS_K_YLEVEL, bytes);
+                hipLaunchKernelGGL((k_ylevel<T, LG, TXV, PRO, PAIR>), grid, dim3(NT), 0, c->stream, I(), P(), sP(),
+                                   c->fsP, fm, nf, pro, I(), sI(), c->fs, da, db, c->p.Nx, twy());
+            })
+        });
+        return HOS_OK;
+    }
+
+    // RHS with each level's y c2r fused with the next level's y r2c:
+    // xinv, ylevel, [xfwdinv, ylevel] x (M-1), xfwd_rhs
     int rhs(const cx<double> *Y, const RkArgs &rk)
     {
-        int rc = w_levels(Y);
-        if (rc) return rc;
-        ProRhs<T> pro{phys(), c->ix, comp()};
-        rc = yr2c_pair(pro, 0, 1, c->nP);
+        const HosIdx &ix = c->ix;
+        const int M = c->M;
+        std::vector<XinvOp> ops;
+        FieldMap fm{};
+        int nf = level0(ops, fm);
+        int rc = xinv(Y, ops);
+        for (int lev = 1; lev <= M && !rc; ++lev) {
+            // fm holds the nf fields of the previous x pass
+            if (lev < M) {
+                ProPhi<T> pro{phys(), ix, lev, comp()};
+                rc = ylevel<ProPhi<T>, false>(fm, nf, pro, 0, 0, lev + 1);
+                const int nd = M - lev;
+                if (!rc) rc = xfwdinv(0, 1, nd);
+                fm = FieldMap{};
+                for (int j = 1; j <= nd; ++j) {
+                    fm.src[j - 1] = j;
+                    fm.dst[j - 1] = ix.d[lev + 1][j];
+                }
+                nf = nd;
+            } else {
+                ProRhs<T> pro{phys(), ix, comp()};
+                rc = ylevel<ProRhs<T>, true>(fm, nf, pro, 0, 1, c->nP);
+            }
+        }
         if (!rc) rc = xfwd_rhs(0, 1, Y, rk);
         return rc;
     }
@@ -1166,9 +1230,7 @@ template <class T>
 static int ifft_impl(hos_ctx *c)
 {
     Eng<T> e{c, 1};
-    XinvOps ops{};
-    ops.o[0][0] = {0, OP_ID, 0, 0};
-    ops.n[0] = 1;
+    const std::vector<XinvOp> ops = {{0, OP_ID, 0, 0}};
     FieldMap fm{};
     fm.src[0] = 0;
     fm.dst[0] = 0;
@@ -1198,12 +1260,7 @@ template <class T>
 static int mult_impl(hos_ctx *c)
 {
     Eng<T> e{c, 1};
-    XinvOps ops{};
-    ops.o[0][0] = {0, OP_ID, 0, 0};
-    ops.o[1][0] = {1, OP_ID, 0, 1};
-    ops.n[0] = ops.n[1] = 1;
-    ops.src[0] = 0;
-    ops.src[1] = 1;
+    const std::vector<XinvOp> ops = {{0, OP_ID, 0, 0}, {1, OP_ID, 0, 1}};
     FieldMap fm{};
     fm.src[0] = 0;
     fm.dst[0] = 0;

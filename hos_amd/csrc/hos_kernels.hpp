@@ -58,11 +58,14 @@ struct XinvOp {
     int pow;  // d^pow/dz^pow for OP_DZ
     int dst;  // intermediate index
 };
-// outputs grouped by source field: group g reads its source row once
+// outputs grouped by source field: group g reads its source row once and
+// forms up to MAXPERGRP outputs from it (one group per blockIdx.y)
+constexpr int MAXGRP = 8, MAXPERGRP = 4;
 struct XinvOps {
-    XinvOp o[2][MAXOPS];
-    int n[2];
-    int src[2];
+    XinvOp o[MAXGRP][MAXPERGRP];
+    int n[MAXGRP];
+    int src[MAXGRP];
+    int ngroups;
 };
 
 // XCD-aware tile order: the 8 XCDs take workgroups round-robin, so give
@@ -151,15 +154,16 @@ struct FieldMap {
     int dst[MAXOPS];
 };
 
-template <class T, int LOGNY, int TX>
-__global__ __launch_bounds__(NT) void k_yc2r(const cx<T> *__restrict__ I, long sI, long fsI, T *__restrict__ P,
-                                             long sP, long fsP, FieldMap fm, int nx,
-                                             const cx<T> *__restrict__ twy)
+// c2r along y of x-rows x0 .. x0+2*NPR-1 of one field: the tile of the
+// x-intermediate src [Nk][nx] is read transposed, rows x0+2p and x0+2p+1 are
+// the real and imaginary part of one complex inverse FFT, written to dst
+// [nx][NY].  buf holds NPR*NY complex values.
+template <class T, int LOGNY, int NPR>
+__device__ __forceinline__ void c2r_tile(const cx<T> *__restrict__ src, T *__restrict__ dst, int x0, int nx,
+                                         cx<T> *buf, const cx<T> *__restrict__ twy)
 {
-    constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1, NPR = TX / 2;
-    __shared__ cx<T> buf[NPR * NY];
-    const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TX;
-    const cx<T> *src = I + blockIdx.z * sI + fm.src[blockIdx.y] * fsI;
+    constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1;
+    __syncthreads();  // buf may still be read by a previous tile's store loop
     for (int e = threadIdx.x; e < NK * NPR; e += NT) {
         const int p = e % NPR, ky = e / NPR;
         cx<T> A = src[(long)ky * nx + x0 + 2 * p];
@@ -172,13 +176,24 @@ __global__ __launch_bounds__(NT) void k_yc2r(const cx<T> *__restrict__ I, long s
         if (ky > 0 && ky < NY / 2) buf[p * NY + NY - ky] = {A.x + B.y, B.x - A.y};
     }
     lds_fft<T, LOGNY, NPR, NT, true>(buf, twy);
-    T *dst = P + blockIdx.z * sP + fm.dst[blockIdx.y] * fsP;
     for (int e = threadIdx.x; e < NPR * NY; e += NT) {
         const int p = e / NY, y = e % NY;
         const cx<T> z = buf[e];
         dst[(long)(x0 + 2 * p) * NY + y] = z.x;
         dst[(long)(x0 + 2 * p + 1) * NY + y] = z.y;
     }
+}
+
+template <class T, int LOGNY, int TX>
+__global__ __launch_bounds__(NT) void k_yc2r(const cx<T> *__restrict__ I, long sI, long fsI, T *__restrict__ P,
+                                             long sP, long fsP, FieldMap fm, int nx,
+                                             const cx<T> *__restrict__ twy)
+{
+    constexpr int NY = 1 << LOGNY, NPR = TX / 2;
+    __shared__ cx<T> buf[NPR * NY];
+    const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TX;
+    c2r_tile<T, LOGNY, NPR>(I + blockIdx.z * sI + fm.src[blockIdx.y] * fsI,
+                            P + blockIdx.z * sP + fm.dst[blockIdx.y] * fsP, x0, nx, buf, twy);
 }
 
 // ---------------------------------------------------------------------------
@@ -338,17 +353,15 @@ struct ProZvel {
 };
 
 // ---------------------------------------------------------------------------
-// y r2c pass, one output field: TX x-rows, rows x0+2p / x0+2p+1 packed as the
-// real / imaginary part of one complex FFT of length Ny.
-// grid (Nx/TX, 1, members)
-template <class T, int LOGNY, int TX, class PRO>
-__global__ __launch_bounds__(NT) void k_yr2c(PRO pro, cx<T> *__restrict__ I, long sI, long fsI, int dst, int nx,
-                                             const cx<T> *__restrict__ twy)
+// r2c along y of x-rows x0 .. x0+2*NPR-1 of the field the prologue computes
+// on the fly: rows x0+2p and x0+2p+1 are the real and imaginary part of one
+// complex FFT; the half spectra are written transposed into out [Nk][nx].
+template <class T, int LOGNY, int NPR, class PRO>
+__device__ __forceinline__ void r2c_tile(const PRO &pro, int m, cx<T> *__restrict__ out, int x0, int nx,
+                                         cx<T> *buf, const cx<T> *__restrict__ twy)
 {
-    constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1, NPR = TX / 2;
-    __shared__ cx<T> buf[NPR * NY];
-    const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TX;
-    const int m = blockIdx.z;
+    constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1, TX = 2 * NPR;
+    __syncthreads();
     // compile-time trip count, unrolled: the loads of several points are in
     // flight together (the prologue reads up to M+1 fields per point)
     constexpr int PER = (NPR * NY + NT - 1) / NT;
@@ -361,7 +374,6 @@ __global__ __launch_bounds__(NT) void k_yr2c(PRO pro, cx<T> *__restrict__ I, lon
         }
     }
     lds_fft<T, LOGNY, NPR, NT, false>(buf, twy);
-    cx<T> *out = I + m * sI + dst * fsI;
     for (int e = threadIdx.x; e < NK * TX; e += NT) {
         const int t = e % TX, ky = e / TX, p = t >> 1;
         const cx<T> zk = buf[p * NY + ky];
@@ -375,39 +387,88 @@ __global__ __launch_bounds__(NT) void k_yr2c(PRO pro, cx<T> *__restrict__ I, lon
     }
 }
 
-// y r2c pass, two output fields at the same x (real / imaginary part of one
-// complex FFT).  grid (Nx/TXP, 1, members)
-template <class T, int LOGNY, int TXP, class PRO>
-__global__ __launch_bounds__(NT) void k_yr2c_pair(PRO pro, cx<T> *__restrict__ I, long sI, long fsI, int dstA,
-                                                  int dstB, int nx, const cx<T> *__restrict__ twy)
+// r2c along y of x-rows x0 .. x0+NROW-1 of two fields the prologue computes
+// together (the real and imaginary part of one complex FFT per x-row).
+template <class T, int LOGNY, int NROW, class PRO>
+__device__ __forceinline__ void r2c_pair_tile(const PRO &pro, int m, cx<T> *__restrict__ oa,
+                                              cx<T> *__restrict__ ob, int x0, int nx, cx<T> *buf,
+                                              const cx<T> *__restrict__ twy)
 {
     constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1;
-    __shared__ cx<T> buf[TXP * NY];
-    const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TXP;
-    const int m = blockIdx.z;
-    constexpr int PER = (TXP * NY + NT - 1) / NT;
+    __syncthreads();
+    constexpr int PER = (NROW * NY + NT - 1) / NT;
 #pragma unroll 2
     for (int q = 0; q < PER; ++q) {
         const int e = threadIdx.x + q * NT;
-        if (e < TXP * NY) {
+        if (e < NROW * NY) {
             const int p = e / NY, y = e % NY;
             T a, b;
             pro(m, x0 + p, y, a, b);
             buf[e] = {a, b};
         }
     }
-    lds_fft<T, LOGNY, TXP, NT, false>(buf, twy);
-    cx<T> *oa = I + m * sI + dstA * fsI;
-    cx<T> *ob = I + m * sI + dstB * fsI;
-    for (int e = threadIdx.x; e < NK * TXP * 2; e += NT) {
-        const int t = e % (2 * TXP), ky = e / (2 * TXP);
-        const int p = t % TXP, which = t / TXP;
+    lds_fft<T, LOGNY, NROW, NT, false>(buf, twy);
+    for (int e = threadIdx.x; e < NK * NROW * 2; e += NT) {
+        const int t = e % (2 * NROW), ky = e / (2 * NROW);
+        const int p = t % NROW, which = t / NROW;
         const cx<T> zk = buf[p * NY + ky];
         const cx<T> zm = buf[p * NY + ((NY - ky) & (NY - 1))];
         if (which == 0)
             oa[(long)ky * nx + x0 + p] = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
         else
             ob[(long)ky * nx + x0 + p] = {T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x)};
+    }
+}
+
+// y r2c pass, one output field.  grid (Nx/TX, 1, members)
+template <class T, int LOGNY, int TX, class PRO>
+__global__ __launch_bounds__(NT) void k_yr2c(PRO pro, cx<T> *__restrict__ I, long sI, long fsI, int dst, int nx,
+                                             const cx<T> *__restrict__ twy)
+{
+    constexpr int NY = 1 << LOGNY, NPR = TX / 2;
+    __shared__ cx<T> buf[NPR * NY];
+    const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TX;
+    const int m = blockIdx.z;
+    r2c_tile<T, LOGNY, NPR, PRO>(pro, m, I + m * sI + dst * fsI, x0, nx, buf, twy);
+}
+
+// y r2c pass, two output fields at the same x.  grid (Nx/TXP, 1, members)
+template <class T, int LOGNY, int TXP, class PRO>
+__global__ __launch_bounds__(NT) void k_yr2c_pair(PRO pro, cx<T> *__restrict__ I, long sI, long fsI, int dstA,
+                                                  int dstB, int nx, const cx<T> *__restrict__ twy)
+{
+    constexpr int NY = 1 << LOGNY;
+    __shared__ cx<T> buf[TXP * NY];
+    const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TXP;
+    const int m = blockIdx.z;
+    r2c_pair_tile<T, LOGNY, TXP, PRO>(pro, m, I + m * sI + dstA * fsI, I + m * sI + dstB * fsI, x0, nx, buf, twy);
+}
+
+// One W level in a single pass over x-tiles: c2r along y of the fields the
+// previous x pass produced (written to P), then, for the same x-rows, the
+// prologue of the next level reads them back and its y r2c is written to I.
+// PAIR selects the two-field RHS prologue.  grid (Nx/TX, 1, members)
+template <class T, int LOGNY, int TX, class PRO, bool PAIR>
+__global__ __launch_bounds__(NT) void k_ylevel(const cx<T> *Iin, T *P, long sP,
+                                               long fsP, FieldMap fm, int nf, PRO pro, cx<T> *Iout, long sI,
+                                               long fsI, int dstA, int dstB, int nx,
+                                               const cx<T> *__restrict__ twy)
+{
+    constexpr int NY = 1 << LOGNY, NPR = TX / 2;
+    __shared__ cx<T> buf[NPR * NY];
+    const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TX;
+    const int m = blockIdx.z;
+    for (int f = 0; f < nf; ++f)
+        c2r_tile<T, LOGNY, NPR>(Iin + m * sI + fm.src[f] * fsI, P + m * sP + fm.dst[f] * fsP, x0, nx, buf, twy);
+    // the prologue reads rows this workgroup has just written
+    __threadfence();
+    if constexpr (PAIR) {
+        r2c_pair_tile<T, LOGNY, NPR, PRO>(pro, m, Iout + m * sI + dstA * fsI, Iout + m * sI + dstB * fsI, x0, nx,
+                                          buf, twy);
+        r2c_pair_tile<T, LOGNY, NPR, PRO>(pro, m, Iout + m * sI + dstA * fsI, Iout + m * sI + dstB * fsI,
+                                          x0 + NPR, nx, buf, twy);
+    } else {
+        r2c_tile<T, LOGNY, NPR, PRO>(pro, m, Iout + m * sI + dstA * fsI, x0, nx, buf, twy);
     }
 }
 

@@ -171,13 +171,15 @@ int hos_last_step_ms(hos_ctx *c, double *ms);
 int hos_step_times(hos_ctx *c, double *ms, int n);
 
 /* Kernel kinds for the per-kernel timing below. */
-#define HOS_K_XINV 0      /* x inverse pass (spectral -> x-intermediate)      */
-#define HOS_K_YC2R 1      /* y c2r pass (x-intermediate -> physical)          */
-#define HOS_K_YR2C 2      /* y r2c pass with a one-field prologue             */
-#define HOS_K_YR2C_PAIR 3 /* y r2c pass with the two-field RHS prologue       */
-#define HOS_K_XFWDINV 4   /* x forward + d^j/dz^j + x inverse (one W level)   */
-#define HOS_K_XFWD_RHS 5  /* x forward pass with the RHS / RK epilogue        */
-#define HOS_K_NKINDS 6
+#define HOS_K_XINV 0        /* x inverse pass (spectral -> x-intermediate)        */
+#define HOS_K_YC2R 1        /* y c2r pass (x-intermediate -> physical)            */
+#define HOS_K_YR2C 2        /* y r2c pass with a one-field prologue               */
+#define HOS_K_YR2C_PAIR 3   /* y r2c pass with the two-field RHS prologue         */
+#define HOS_K_XFWDINV 4     /* x forward + d^j/dz^j + x inverse (one W level)     */
+#define HOS_K_XFWD_RHS 5    /* x forward pass with the RHS / RK epilogue          */
+#define HOS_K_YLEVEL 6      /* y c2r of a level + y r2c of the next phi^(l)       */
+#define HOS_K_YLEVEL_PAIR 7 /* y c2r of the last level + y r2c of the RHS fields  */
+#define HOS_K_NKINDS 8
 
 /* 1: record a HIP event pair around every stream-ordered kernel launch of
  * later hos_step calls (not graph replays) and accumulate, per kernel kind,
