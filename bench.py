@@ -6,12 +6,16 @@ Workload (the single-instance 1024^2 configuration): 1024x1024 grid over a
 HOS order M = 5 (nlevs = 4), JONSWAP gamma = 5, kp*Hs/2 = 0.12, cos^2
 spreading, |k| <= 4 kp, seed 1, classical RK4 with dt = Tp/100.
 
-A step is one RK4 step (4 RHS evaluations) of every member.  The FP32 form
-(FFTs, spectral multiplies and products in FP32 with Kahan-compensated order
-sums; FP64 state, stage combination, linear terms and diagnostics) is timed
-first, then the plain FP64 form on the same input.  value = grid-point RK4
-steps/s of the FP32 form, state resident in HBM, timed between two device
-synchronisations.
+A step is one RK4 step (4 RHS evaluations) of every member.  Timed in order:
+  1. the FP32 batched form: --members (default 8) realisations of the
+     workload advanced in the same launches (FFTs, spectral multiplies and
+     products in FP32 with Kahan-compensated order sums; FP64 state, stage
+     combination, linear terms and diagnostics), one captured graph per step;
+  2. the same FP32 form on one instance;
+  3. the plain FP64 form on one instance, same input.
+value = grid-point RK4 steps/s of (1): members * Nx * Ny * steps / wall time
+of the timed steps, state resident in HBM, between two device
+synchronisations.  p50/p99 are per-step times from HIP events.
 
 Prints one JSON line.
 """
@@ -110,13 +114,15 @@ def cpu_baseline(w, dt, u0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--members", type=int, default=1)
+    ap.add_argument("--members", type=int, default=8, help="1024^2 realisations in the FP32 batched form")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--calib", action="store_true", help="only the plain copy kernel (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); members of an ensemble "
@@ -125,19 +131,32 @@ def main():
 
     import hos_amd as hos
 
+    if args.calib:
+        # 256 MiB copied 10 times: the per-dispatch counter reading of a known
+        # 512 MiB (read + write) of traffic
+        gbps = hos.copy_bandwidth(256 << 20, 10)
+        print(json.dumps({"copy_bytes_per_dispatch": 2 * (256 << 20), "copy_GBps": round(gbps, 1)}))
+        return
+
     w = workload()
     Tp = hos.peak_period(w["g"], w["kp"], w["depth"])
     dt = Tp / 100.0
     u0 = make_state(hos, w)
     print(f"[bench] device: {hos.device_name()}", file=sys.stderr, flush=True)
+    graph = not args.no_graph
 
-    f32 = run_form(hos, w, hos.HOS_FP32, args.members, args.steps, args.warmup, dt, u0,
-                   graph=not args.no_graph, profile_steps=args.profile_steps)
-    print(f"[bench] fp32: {f32['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    f32 = run_form(hos, w, hos.HOS_FP32, args.members, args.steps, args.warmup, dt, u0, graph=graph,
+                   profile_steps=args.profile_steps)
+    print(f"[bench] fp32 x{args.members}: {f32['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    f1 = None
+    if not args.no_single and args.members != 1:
+        f1 = run_form(hos, w, hos.HOS_FP32, 1, args.steps, args.warmup, dt, u0, graph=graph)
+        print(f"[bench] fp32 x1: {f1['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     f64 = None
     if not args.no_fp64:
-        f64 = run_form(hos, w, hos.HOS_FP64, 1, args.steps, args.warmup, dt, u0, graph=not args.no_graph)
-        print(f"[bench] fp64: {f64['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+        f64 = run_form(hos, w, hos.HOS_FP64, 1, args.steps, args.warmup, dt, u0, graph=graph)
+        print(f"[bench] fp64 x1: {f64['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    copy_gbps = hos.copy_bandwidth(256 << 20, 10)
 
     # dominant kernel: largest summed duration in the profiled pass
     ks = f32.get("kstats", {})
@@ -153,12 +172,17 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
             "avg_launch_us": round(avg_ms * 1e3, 2), "bytes_per_launch": avg_bytes,
             "kernel_share": round(k["ms"] / sum(v["ms"] for v in ks.values()), 3),
+            "practical_copy_GBps": round(copy_gbps, 1),
             "per_kernel": {n: {"launches": v["launches"],
                                "avg_us": round(1e3 * v["ms"] / max(v["launches"], 1), 2),
                                "GBps": round(v["bytes"] / max(v["ms"], 1e-12) / 1e6, 1)}
                            for n, v in ks.items() if v["launches"]},
         }
     cpu = None if args.no_cpu else cpu_baseline(w, dt, u0)
+
+    def line(f):
+        return {"value": round(f["value"], 1), "ms_per_step": round(f["ms_per_step"], 4),
+                "p50_ms": round(f["p50_ms"], 4), "p99_ms": round(f["p99_ms"], 4)}
 
     res = {
         "metric": "grid-pt RK4 steps/s, M=5 1024^2, FP32 batched vs FP64 plain; p50/p99 ms",
@@ -174,21 +198,25 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
-        "data": "synthetic seeded JONSWAP sea state (hos_ic_jonswap)",
+        "data": "synthetic seeded JONSWAP sea states (hos_ic_jonswap), resident in HBM",
         "config": {
-            "workload": "1024x1024 grid, 32x32 peak wavelengths, kp*h=1, M=5, JONSWAP gamma=5 kp*Hs/2=0.12 "
-                        "seed 1, RK4 dt=Tp/100",
-            "members": args.members, "graph": not args.no_graph, "y_tiles_c2r_r2c_pair": f32["tiles"],
+            "workload": f"FP32 batched form: {args.members} realisations of the 1024x1024 grid over 32x32 peak "
+                        "wavelengths, kp*h=1, M=5, JONSWAP gamma=5 kp*Hs/2=0.12 seed 1, RK4 dt=Tp/100, all "
+                        "members in the same launches; compared with the plain FP64 single instance",
+            "members": args.members, "graph": graph, "y_tiles_c2r_r2c_pair": f32["tiles"],
             "form": "FP32 FFTs/products with Kahan order sums; FP64 state, RK combination, linear terms",
             "parallelism": "single GPU",
         },
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if f1:
+        res["fp32_single"] = line(f1)
     if f64:
-        res["fp64_plain"] = {"value": round(f64["value"], 1), "ms_per_step": round(f64["ms_per_step"], 4),
-                             "p50_ms": round(f64["p50_ms"], 4), "p99_ms": round(f64["p99_ms"], 4)}
-        res["speedup_fp32_vs_fp64"] = round(f32["value"] / f64["value"], 3)
+        res["fp64_plain"] = line(f64)
+        res["speedup_fp32_batched_vs_fp64_plain"] = round(f32["value"] / f64["value"], 3)
+        if f1:
+            res["speedup_fp32_single_vs_fp64_plain"] = round(f1["value"] / f64["value"], 3)
         d = f32["state"][0] - f64["state"][0]
         res["eta_rel_l2_fp32_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(f64["state"][0]))
         res["steps_compared"] = args.warmup + args.steps

@@ -43,13 +43,13 @@ EXPORTS = [
     "hos_last_error", "hos_device_name", "hos_num_members", "hos_bytes_per_point_step", "hos_step_times",
     "hos_profile", "hos_kernel_stats", "hos_spectral_op", "hos_mult", "hos_hamiltonian",
     "hos_set_active", "hos_get_active", "hos_set_member", "hos_copy_member", "hos_step_async", "hos_wait",
-    "hos_autotune",
+    "hos_autotune", "hos_copy_bandwidth",
 ]
 HOS_OP_DX, HOS_OP_DY, HOS_OP_DZ, HOS_OP_SUM, HOS_OP_DEALIAS = 1, 2, 3, 4, 5
 # include/hos_ic.h
 IC_EXPORTS = ["hos_ic_jonswap", "hos_ic_uniform", "hos_ic_peak_period", "hos_dealias_state"]
 
-KERNEL_KINDS = ["xinv", "yc2r", "yr2c", "yr2c_pair", "xfwdinv", "xfwd_rhs", "ylevel", "ylevel_pair"]
+KERNEL_KINDS = ["xinv", "yc2r", "yr2c", "yr2c_pair", "xfwdinv", "xfwd_rhs"]
 
 
 class Jonswap(ctypes.Structure):
@@ -150,6 +150,15 @@ def device_name() -> str:
     b = ctypes.create_string_buffer(256)
     _check(load().hos_device_name(b, 256))
     return b.value.decode()
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20) -> float:
+    """GB/s (read + write) of a plain device copy of nbytes (hos_copy_bandwidth)."""
+    lib = load()
+    lib.hos_copy_bandwidth.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    v = ctypes.c_double()
+    _check(lib.hos_copy_bandwidth(nbytes, reps, ctypes.byref(v)))
+    return v.value
 
 
 def footprint(p: Params, nmembers: int = 1) -> int:

@@ -492,9 +492,12 @@ struct Eng {
     bool comp() const { return c->f32 && c->p.compensated; }
 
     // outputs list[i] (dst = intermediate index), grouped by source into
-    // chunks of at most `per` outputs, one chunk per blockIdx.y
-    int xinv(const cx<double> *Y, const std::vector<XinvOp> &list, int per = 3)
+    // chunks of at most `per` outputs, one chunk per blockIdx.y.  Few rows in
+    // flight (one member): small chunks for more workgroups; many members:
+    // one chunk per source, so each source row is read once.
+    int xinv(const cx<double> *Y, const std::vector<XinvOp> &list, int per = 0)
     {
+        if (per <= 0) per = ((long)nm * c->Nk >= 4096) ? MAXPERGRP : 3;
         XinvOps ops{};
         for (int s = 0; s < 2; ++s) {
             int k = 0;
@@ -643,52 +646,16 @@ struct Eng {
         return rc;
     }
 
-    // c2r of nf fields (fm) then the next prologue's r2c, on the same x-tiles
-    template <class PRO, bool PAIR>
-    int ylevel(const FieldMap &fm, int nf, const PRO &pro, int da, int db, int nin)
-    {
-        const int tx = c->tx[PAIR ? HOS_TILE_PAIR : HOS_TILE_R2C];
-        const double bytes = (double)nm * sizeof(T) *
-                             (nf * (2.0 * c->fs + c->fsP) + nin * (double)c->fsP + (PAIR ? 4.0 : 2.0) * c->fs);
-        LOGN_SWITCH(c->logny, {
-            TX_SWITCH(T, tx, {
-                dim3 grid(c->p.Nx / TXV, 1, nm);
-                Prof pf(c, PAIR ? HOS_K_YLEVEL_PAIR : HOS_K_YLEVEL, bytes);
-                hipLaunchKernelGGL((k_ylevel<T, LG, TXV, PRO, PAIR>), grid, dim3(NT), 0, c->stream, I(), P(), sP(),
-                                   c->fsP, fm, nf, pro, I(), sI(), c->fs, da, db, c->p.Nx, twy());
-            })
-        });
-        return HOS_OK;
-    }
-
-    // RHS with each level's y c2r fused with the next level's y r2c:
-    // xinv, ylevel, [xfwdinv, ylevel] x (M-1), xfwd_rhs
+    // RHS: the W levels (each field its own workgroups), the two RHS fields'
+    // y r2c, and the x forward pass with the RK epilogue.  (Fusing a level's
+    // y c2r with the next y r2c in one kernel serialises the fields of a
+    // level inside a workgroup and measured slower.)
     int rhs(const cx<double> *Y, const RkArgs &rk)
     {
-        const HosIdx &ix = c->ix;
-        const int M = c->M;
-        std::vector<XinvOp> ops;
-        FieldMap fm{};
-        int nf = level0(ops, fm);
-        int rc = xinv(Y, ops);
-        for (int lev = 1; lev <= M && !rc; ++lev) {
-            // fm holds the nf fields of the previous x pass
-            if (lev < M) {
-                ProPhi<T> pro{phys(), ix, lev, comp()};
-                rc = ylevel<ProPhi<T>, false>(fm, nf, pro, 0, 0, lev + 1);
-                const int nd = M - lev;
-                if (!rc) rc = xfwdinv(0, 1, nd);
-                fm = FieldMap{};
-                for (int j = 1; j <= nd; ++j) {
-                    fm.src[j - 1] = j;
-                    fm.dst[j - 1] = ix.d[lev + 1][j];
-                }
-                nf = nd;
-            } else {
-                ProRhs<T> pro{phys(), ix, comp()};
-                rc = ylevel<ProRhs<T>, true>(fm, nf, pro, 0, 1, c->nP);
-            }
-        }
+        int rc = w_levels(Y);
+        if (rc) return rc;
+        ProRhs<T> pro{phys(), c->ix, comp()};
+        rc = yr2c_pair(pro, 0, 1, c->nP);
         if (!rc) rc = xfwd_rhs(0, 1, Y, rk);
         return rc;
     }
@@ -1331,6 +1298,41 @@ extern "C" int hos_hamiltonian(hos_ctx *c, const double *heta, const double *het
 }
 
 // ---------------------------------------------------------------------------
+// plain device copy of a known size (calibrates the HBM counters and gives
+// the practical copy bandwidth next to the 8 TB/s peak)
+extern "C" int hos_copy_bandwidth(size_t bytes, int reps, double *gbps)
+{
+    if (!gbps || reps < 1 || bytes < 4096) return fail(HOS_E_INVALID, "bad argument");
+    const long n = (long)(bytes / sizeof(cx<double>));
+    void *a = nullptr, *b = nullptr;
+    HIP_TRY(hipMalloc(&a, n * sizeof(cx<double>)));
+    if (hipMalloc(&b, n * sizeof(cx<double>)) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(HOS_E_NOMEM, "copy buffers");
+    }
+    (void)hipMemset(a, 0, n * sizeof(cx<double>));
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const int grid = 8192;
+    hipLaunchKernelGGL(k_copy_c64, dim3(grid), dim3(NT), 0, nullptr, (const cx<double> *)a, (cx<double> *)b, n);
+    (void)hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(k_copy_c64, dim3(grid), dim3(NT), 0, nullptr, (const cx<double> *)a, (cx<double> *)b,
+                           n);
+    (void)hipEventRecord(e1, nullptr);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(HOS_E_DEVICE, "copy timing: %s", hipGetErrorString(e));
+    *gbps = 2.0 * n * sizeof(cx<double>) * reps / (ms * 1e-3) / 1e9;
+    return HOS_OK;
+}
+
 extern "C" int hos_device_name(char *buf, int len)
 {
     if (!buf || len <= 0) return fail(HOS_E_INVALID, "bad buffer");

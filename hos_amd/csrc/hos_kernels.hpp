@@ -60,7 +60,7 @@ struct XinvOp {
 };
 // outputs grouped by source field: group g reads its source row once and
 // forms up to MAXPERGRP outputs from it (one group per blockIdx.y)
-constexpr int MAXGRP = 8, MAXPERGRP = 4;
+constexpr int MAXGRP = 8, MAXPERGRP = 12;
 struct XinvOps {
     XinvOp o[MAXGRP][MAXPERGRP];
     int n[MAXGRP];
@@ -442,34 +442,6 @@ __global__ __launch_bounds__(NT) void k_yr2c_pair(PRO pro, cx<T> *__restrict__ I
     const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TXP;
     const int m = blockIdx.z;
     r2c_pair_tile<T, LOGNY, TXP, PRO>(pro, m, I + m * sI + dstA * fsI, I + m * sI + dstB * fsI, x0, nx, buf, twy);
-}
-
-// One W level in a single pass over x-tiles: c2r along y of the fields the
-// previous x pass produced (written to P), then, for the same x-rows, the
-// prologue of the next level reads them back and its y r2c is written to I.
-// PAIR selects the two-field RHS prologue.  grid (Nx/TX, 1, members)
-template <class T, int LOGNY, int TX, class PRO, bool PAIR>
-__global__ __launch_bounds__(NT) void k_ylevel(const cx<T> *Iin, T *P, long sP,
-                                               long fsP, FieldMap fm, int nf, PRO pro, cx<T> *Iout, long sI,
-                                               long fsI, int dstA, int dstB, int nx,
-                                               const cx<T> *__restrict__ twy)
-{
-    constexpr int NY = 1 << LOGNY, NPR = TX / 2;
-    __shared__ cx<T> buf[NPR * NY];
-    const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TX;
-    const int m = blockIdx.z;
-    for (int f = 0; f < nf; ++f)
-        c2r_tile<T, LOGNY, NPR>(Iin + m * sI + fm.src[f] * fsI, P + m * sP + fm.dst[f] * fsP, x0, nx, buf, twy);
-    // the prologue reads rows this workgroup has just written
-    __threadfence();
-    if constexpr (PAIR) {
-        r2c_pair_tile<T, LOGNY, NPR, PRO>(pro, m, Iout + m * sI + dstA * fsI, Iout + m * sI + dstB * fsI, x0, nx,
-                                          buf, twy);
-        r2c_pair_tile<T, LOGNY, NPR, PRO>(pro, m, Iout + m * sI + dstA * fsI, Iout + m * sI + dstB * fsI,
-                                          x0 + NPR, nx, buf, twy);
-    } else {
-        r2c_tile<T, LOGNY, NPR, PRO>(pro, m, Iout + m * sI + dstA * fsI, x0, nx, buf, twy);
-    }
 }
 
 // ---------------------------------------------------------------------------

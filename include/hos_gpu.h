@@ -177,9 +177,7 @@ int hos_step_times(hos_ctx *c, double *ms, int n);
 #define HOS_K_YR2C_PAIR 3   /* y r2c pass with the two-field RHS prologue         */
 #define HOS_K_XFWDINV 4     /* x forward + d^j/dz^j + x inverse (one W level)     */
 #define HOS_K_XFWD_RHS 5    /* x forward pass with the RHS / RK epilogue          */
-#define HOS_K_YLEVEL 6      /* y c2r of a level + y r2c of the next phi^(l)       */
-#define HOS_K_YLEVEL_PAIR 7 /* y c2r of the last level + y r2c of the RHS fields  */
-#define HOS_K_NKINDS 8
+#define HOS_K_NKINDS 6
 
 /* 1: record a HIP event pair around every stream-ordered kernel launch of
  * later hos_step calls (not graph replays) and accumulate, per kernel kind,
@@ -210,6 +208,11 @@ int hos_set_tile(hos_ctx *c, int kind, int tx);
 int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out);
 
 const char *hos_last_error(void);
+
+/* Plain device copy (k_copy_c64) of `bytes`, timed with HIP events over reps
+ * launches: the practical copy bandwidth in GB/s (bytes read + written), and
+ * the copy kernel of known size that calibrates the HBM byte counters. */
+int hos_copy_bandwidth(size_t bytes, int reps, double *gbps);
 
 /* Introspection used by tests and the benchmark. */
 int hos_device_name(char *buf, int len);

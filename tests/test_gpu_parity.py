@@ -193,6 +193,26 @@ def test_autotune_restores_state_and_tiles_agree(hos):
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
 
 
+def test_mean_level_correction_and_volume(hos):
+    """mean_level=1 holds eta-hat(0,0) at its initial value (FP64); the
+    reported volume is Re eta-hat(0,0) at the start of the last step."""
+    Nx, Ny, nlevs = 64, 64, 3
+    print("mean level", flush=True)
+    u = random_state(Nx, Ny, nlevs, amp=0.08, seed=17)
+    u[0, 0, 0] = 0.0123
+    for ml in (0, 1):
+        s = hos.Solver(hos.params(Nx=Nx, Ny=Ny, nlevs=nlevs, precision=1, mean_level=ml))
+        s.set_state(u)
+        s.step(5, 0.02)
+        st = s.get_state()
+        if ml:
+            assert st[0, 0, 0] == 0.0123
+        H, vol = s.energy()
+        assert abs(vol - 0.0123) < 1e-6
+        s.step(1, 0.02)
+        assert s.energy()[1] == st[0, 0, 0].real
+
+
 def test_graph_replay_matches_eager(hos):
     Nx, Ny, nlevs = 64, 64, 4
     print("graph", flush=True)
