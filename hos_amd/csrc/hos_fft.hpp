@@ -1,10 +1,12 @@
 // hos_fft.hpp -- device building blocks of the batched 2-D real FFTs (gfx950).
 //
 // A row FFT of length N = 2^LOGN runs in LDS as a Stockham autosort sequence
-// of radix-4 stages (one radix-2 stage for odd LOGN; radix-2 only for N = 8),
-// all threads of the workgroup cooperating on NROWS rows at once.  Each stage
-// reads its R inputs into registers, barriers, and writes R outputs in place,
-// so one buffer of NROWS*N complex values is all the LDS a transform needs.
+// of radix-16/8/4/2 stages (fft_stage_bits: 1024 = 16*16*4 when enough rows
+// share the workgroup, 8*8*4*4 for one row), all threads of the workgroup
+// cooperating on NROWS rows at once.  Each stage reads its R inputs into
+// registers, does the R-point DFT there (radix 16 = 4 x 4), barriers, and
+// writes R outputs in place, so one buffer of NROWS*N complex values is all
+// the LDS a transform needs and each stage is one LDS round trip.
 // Twiddles come from a per-length table tw[k] = exp(-2 pi i k/N) computed on
 // the host in long double; the inverse uses the conjugate.  Transforms are
 // unnormalised (the 1/(Nx*Ny) of fft_2d is applied by the r2c x-pass).
@@ -28,34 +30,115 @@ __device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) { return {a.x + b.x, a.y
 template <class T>
 __device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { return {a.x - b.x, a.y - b.y}; }
 
-// stage plan: radix of stage s for length 2^logn.  The first and last stage
-// always share a radix.
-__host__ __device__ constexpr int fft_nstages(int logn)
+// Stage plan for NROWS rows of length 2^logn: radix 16, 8, 4 or 2 per stage,
+// the largest radix that keeps at least min(128, NROWS*N/4) butterflies per
+// stage (so most of a 256-thread workgroup stays busy) and does not leave a
+// single radix-2 bit behind.  Fewer stages = fewer LDS round trips.
+__host__ __device__ constexpr int fft_stage_bits(int logn, int nrows, int s)
 {
-    return logn == 0 ? 0 : logn == 1 ? 1 : logn == 3 ? 3 : (logn % 2 == 0 ? logn / 2 : (logn - 1) / 2 + 1);
+    const long total = (long)nrows << logn;
+    const long thr = total / 4 < 128 ? total / 4 : 128;
+    int rem = logn, bits = 0;
+    for (int i = 0; i <= s; ++i) {
+        int b = rem == 1 ? 1 : 2;
+        if (4 <= rem && rem - 4 != 1 && (total >> 4) >= thr)
+            b = 4;
+        else if (3 <= rem && rem - 3 != 1 && (total >> 3) >= thr)
+            b = 3;
+        if ((rem == 2 || rem == 3) && b < rem && (total >> rem) >= thr) b = rem;
+        bits = b;
+        rem -= b;
+        if (rem <= 0 && i < s) return 0;
+    }
+    return bits;
 }
-__host__ __device__ constexpr int fft_radix(int logn, int s)
+__host__ __device__ constexpr int fft_nstages(int logn, int nrows)
 {
-    return logn == 1 ? 2 : logn == 3 ? 2 : (logn % 2 == 0 ? 4 : (s == 1 ? 2 : 4));
+    int n = 0;
+    while (fft_stage_bits(logn, nrows, n) > 0) ++n;
+    return n;
 }
 
+// e^{-2 pi i m/16}, m = 0..15 (cos and sin of multiples of pi/8)
+template <class T>
+__device__ __forceinline__ cx<T> w16(int m)
+{
+    constexpr double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173, h = 0.70710678118654752440;
+    constexpr double re[16] = {1, c1, h, s1, 0, -s1, -h, -c1, -1, -c1, -h, -s1, 0, s1, h, c1};
+    constexpr double im[16] = {0, -s1, -h, -c1, -1, -c1, -h, -s1, 0, s1, h, c1, 1, c1, h, s1};
+    return {T(re[m & 15]), T(im[m & 15])};
+}
+
+template <class T, bool INV>
+__device__ __forceinline__ void dft4(cx<T> &a0, cx<T> &a1, cx<T> &a2, cx<T> &a3)
+{
+    const cx<T> t0 = cadd(a0, a2), t1 = csub(a0, a2);
+    const cx<T> t2 = cadd(a1, a3), d = csub(a1, a3);
+    // forward: t3 = -i*d ; inverse: t3 = +i*d
+    const cx<T> t3 = INV ? cx<T>{-d.y, d.x} : cx<T>{d.y, -d.x};
+    a0 = cadd(t0, t2);
+    a1 = cadd(t1, t3);
+    a2 = csub(t0, t2);
+    a3 = csub(t1, t3);
+}
+
+template <class T, bool INV>
+__device__ __forceinline__ cx<T> tw16(cx<T> v, int m)
+{
+    cx<T> w = w16<T>(m);
+    if (INV) w.y = -w.y;
+    return cmul(v, w);
+}
+
+// in-register DFT of size R, natural order in and out
 template <class T, int R, bool INV>
 __device__ __forceinline__ void dft_small(cx<T> *v)
 {
     if constexpr (R == 2) {
-        cx<T> a = v[0], b = v[1];
+        const cx<T> a = v[0], b = v[1];
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+        dft4<T, INV>(v[0], v[1], v[2], v[3]);
+    } else if constexpr (R == 8) {
+        // even / odd halves, then X[k] = E[k] + w8^k O[k], X[k+4] = E[k] - w8^k O[k]
+        cx<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+        cx<T> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+        dft4<T, INV>(e0, e1, e2, e3);
+        dft4<T, INV>(o0, o1, o2, o3);
+        o1 = tw16<T, INV>(o1, 2);
+        o2 = tw16<T, INV>(o2, 4);
+        o3 = tw16<T, INV>(o3, 6);
+        v[0] = cadd(e0, o0);
+        v[4] = csub(e0, o0);
+        v[1] = cadd(e1, o1);
+        v[5] = csub(e1, o1);
+        v[2] = cadd(e2, o2);
+        v[6] = csub(e2, o2);
+        v[3] = cadd(e3, o3);
+        v[7] = csub(e3, o3);
     } else {
-        static_assert(R == 4, "radix");
-        cx<T> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
-        cx<T> t2 = cadd(v[1], v[3]), d = csub(v[1], v[3]);
-        // forward: t3 = -i*d ; inverse: t3 = +i*d
-        cx<T> t3 = INV ? cx<T>{-d.y, d.x} : cx<T>{d.y, -d.x};
-        v[0] = cadd(t0, t2);
-        v[1] = cadd(t1, t3);
-        v[2] = csub(t0, t2);
-        v[3] = csub(t1, t3);
+        static_assert(R == 16, "radix");
+        // 16 = 4 x 4: DFT4 over n1 for each n2, twiddle w16^(n2 k1), DFT4 over n2
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) dft4<T, INV>(v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]);
+        // v[n2 + 4 k1] now holds A[n2][k1]
+#pragma unroll
+        for (int n2 = 1; n2 < 4; ++n2)
+#pragma unroll
+            for (int k1 = 1; k1 < 4; ++k1) v[n2 + 4 * k1] = tw16<T, INV>(v[n2 + 4 * k1], n2 * k1);
+        cx<T> o[16];
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            cx<T> b0 = v[4 * k1], b1 = v[1 + 4 * k1], b2 = v[2 + 4 * k1], b3 = v[3 + 4 * k1];
+            dft4<T, INV>(b0, b1, b2, b3);
+            o[k1] = b0;
+            o[k1 + 4] = b1;
+            o[k1 + 8] = b2;
+            o[k1 + 12] = b3;
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = o[i];
     }
 }
 
@@ -76,15 +159,36 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, const cx<T> *__restrict__ 
 #pragma unroll
             for (int r = 0; r < R; ++r) v[q][r] = src[j + r * NR];
             if constexpr (NS > 1) {
-                // w^r for r = 1..R-1 from one table entry w = exp(-+2 pi i k/(NS R))
+                // w^r, r = 1..R-1, w = exp(-+2 pi i k/(NS R)): powers below 4 by
+                // repeated products of w, higher ones as (w^4)^q * w^b with w^4 a
+                // second table entry, so no power is more than three products
+                // away from a table value.
                 const int k = j % NS;
-                cx<T> w = tw[k * (N / (NS * R))];
-                if (INV) w.y = -w.y;
-                cx<T> wr = w;
+                constexpr int STRIDE = N / (NS * R);
+                cx<T> w1 = tw[k * STRIDE];
+                if (INV) w1.y = -w1.y;
+                cx<T> wb[4];
+                wb[0] = {T(1), T(0)};
+                wb[1] = w1;
+                if constexpr (R > 2) {
+                    wb[2] = cmul(w1, w1);
+                    wb[3] = cmul(wb[2], w1);
+                }
+                cx<T> wq[4];
+                wq[0] = {T(1), T(0)};
+                if constexpr (R > 4) {
+                    cx<T> w4 = tw[4 * k * STRIDE];
+                    if (INV) w4.y = -w4.y;
+                    wq[1] = w4;
+                    if constexpr (R > 8) {
+                        wq[2] = cmul(w4, w4);
+                        wq[3] = cmul(wq[2], w4);
+                    }
+                }
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    v[q][r] = cmul(v[q][r], wr);
-                    if (r + 1 < R) wr = cmul(wr, w);
+                    const cx<T> w = (r >> 2) == 0 ? wb[r & 3] : ((r & 3) == 0 ? wq[r >> 2] : cmul(wq[r >> 2], wb[r & 3]));
+                    v[q][r] = cmul(v[q][r], w);
                 }
             }
             dft_small<T, R, INV>(v[q]);
@@ -108,8 +212,8 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, const cx<T> *__restrict__ 
 template <class T, int LOGN, int NROWS, int NT, bool INV, int S = 0, int NS = 1>
 __device__ __forceinline__ void fft_stages(cx<T> *buf, const cx<T> *__restrict__ tw)
 {
-    if constexpr (S < fft_nstages(LOGN)) {
-        constexpr int R = fft_radix(LOGN, S);
+    if constexpr (S < fft_nstages(LOGN, NROWS)) {
+        constexpr int R = 1 << fft_stage_bits(LOGN, NROWS, S);
         fft_stage<T, R, LOGN, NROWS, NT, INV, NS>(buf, tw);
         fft_stages<T, LOGN, NROWS, NT, INV, S + 1, NS * R>(buf, tw);
     }
