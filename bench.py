@@ -111,6 +111,43 @@ def cpu_baseline(w, dt, u0):
     }
 
 
+KIND_OF = [("k_yr2c_pair", "yr2c_pair"), ("k_yr2c", "yr2c"), ("k_yc2r", "yc2r"), ("k_xfwdinv", "xfwdinv"),
+           ("k_xfwd_rhs", "xfwd_rhs"), ("k_xinv", "xinv"), ("k_copy_c64", "copy")]
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_1024_m5_x8.json")
+
+
+def traffic_summary(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir):
+    """HBM bytes per launch per kernel kind from rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE runs (KiB per dispatch), each scaled by what the counter reads
+    for the plain copy kernel of known size (bench.py --calib)."""
+    import csv
+    from collections import defaultdict
+
+    def per_kind(d):
+        acc = defaultdict(list)
+        for r in csv.DictReader(open(os.path.join(d, "run_counter_collection.csv"))):
+            name = r["Kernel_Name"]
+            kind = next((k for pat, k in KIND_OF if pat in name), None)
+            if kind:
+                acc[kind].append(float(r["Counter_Value"]))
+        return {k: sum(v) / len(v) for k, v in acc.items()}, {k: len(v) for k, v in acc.items()}
+
+    f, nf = per_kind(fetch_dir)
+    w, _ = per_kind(write_dir)
+    cf, _ = per_kind(cal_fetch_dir)
+    cw, _ = per_kind(cal_write_dir)
+    copy_bytes = 256 << 20  # read and written once per copy dispatch
+    sf = copy_bytes / (cf["copy"] * 1024.0)
+    sw = copy_bytes / (cw["copy"] * 1024.0)
+    out = {"fetch_scale": sf, "write_scale": sw, "kernels": {}}
+    for k in f:
+        if k == "copy" or k not in w:
+            continue
+        out["kernels"][k] = {"dispatches": nf[k], "fetch_KiB": f[k], "write_KiB": w[k],
+                             "bytes_per_launch": (f[k] * sf + w[k] * sw) * 1024.0}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,7 +160,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--calib", action="store_true", help="only the plain copy kernel (counter calibration)")
+    ap.add_argument("--traffic", nargs=4, metavar=("FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
+                    help="summarise four rocprofv3 --pmc output folders into " + TRAFFIC_FILE)
     args = ap.parse_args()
+    if args.traffic:
+        t = traffic_summary(*args.traffic)
+        with open(TRAFFIC_FILE, "w") as fh:
+            json.dump(t, fh, indent=1)
+        print(json.dumps(t))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); members of an ensemble "
               "shard across GPUs with no communication, see DESIGN.md", file=sys.stderr)
@@ -167,9 +212,13 @@ def main():
         avg_ms = k["ms"] / max(k["launches"], 1)
         avg_bytes = k["bytes"] / max(k["launches"], 1)
         achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+        traffic = None
+        if args.members == 8 and os.path.exists(TRAFFIC_FILE):
+            tk = json.load(open(TRAFFIC_FILE)).get("kernels", {}).get(name)
+            traffic = round(tk["bytes_per_launch"]) if tk else None
         roof = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": name,
             "avg_launch_us": round(avg_ms * 1e3, 2), "bytes_per_launch": avg_bytes,
             "kernel_share": round(k["ms"] / sum(v["ms"] for v in ks.values()), 3),
             "practical_copy_GBps": round(copy_gbps, 1),
