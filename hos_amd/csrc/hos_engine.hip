@@ -621,6 +621,48 @@ struct Eng {
         return n;
     }
 
+    // y r2c of phi^(lev+1), the prologue instantiated for each level
+    template <int LEV>
+    int phi_level_t()
+    {
+        ProPhi<T, LEV> pro{phys(), c->ix, comp()};
+        return yr2c(pro, 0, LEV + 1);
+    }
+    int phi_level(int lev)
+    {
+        switch (lev) {
+        case 1: return phi_level_t<1>();
+        case 2: return phi_level_t<2>();
+        case 3: return phi_level_t<3>();
+        case 4: return phi_level_t<4>();
+        case 5: return phi_level_t<5>();
+        case 6: return phi_level_t<6>();
+        case 7: return phi_level_t<7>();
+        }
+        static_assert(MAXM == 8, "levels 1..MAXM-1");
+        return fail(HOS_E_INVALID, "level %d out of range", lev);
+    }
+    template <int MO>
+    int rhs_pair_t()
+    {
+        ProRhs<T, MO> pro{phys(), c->ix, comp()};
+        return yr2c_pair(pro, 0, 1, c->nP);
+    }
+    int rhs_pair()
+    {
+        switch (c->M) {
+        case 1: return rhs_pair_t<1>();
+        case 2: return rhs_pair_t<2>();
+        case 3: return rhs_pair_t<3>();
+        case 4: return rhs_pair_t<4>();
+        case 5: return rhs_pair_t<5>();
+        case 6: return rhs_pair_t<6>();
+        case 7: return rhs_pair_t<7>();
+        case 8: return rhs_pair_t<8>();
+        }
+        return fail(HOS_E_INVALID, "order %d out of range", c->M);
+    }
+
     // physical fields of the W expansion from the stage input Y
     int w_levels(const cx<double> *Y)
     {
@@ -632,8 +674,7 @@ struct Eng {
         int rc = xinv(Y, ops);
         if (!rc) rc = yc2r(fm, n);
         for (int lev = 1; lev <= M - 1 && !rc; ++lev) {
-            ProPhi<T> pro{phys(), ix, lev, comp()};
-            rc = yr2c(pro, 0, lev + 1);
+            rc = phi_level(lev);
             const int nd = M - lev;
             if (!rc) rc = xfwdinv(0, 1, nd);
             FieldMap f2{};
@@ -654,8 +695,7 @@ struct Eng {
     {
         int rc = w_levels(Y);
         if (rc) return rc;
-        ProRhs<T> pro{phys(), c->ix, comp()};
-        rc = yr2c_pair(pro, 0, 1, c->nP);
+        rc = rhs_pair();
         if (!rc) rc = xfwd_rhs(0, 1, Y, rk);
         return rc;
     }

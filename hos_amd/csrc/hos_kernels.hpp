@@ -24,6 +24,47 @@ constexpr int NT = 256;   // threads per workgroup
 constexpr int MAXM = 8;   // HOS order M = nlevs+1 up to 8
 constexpr int MAXOPS = 16;
 
+// 16-byte vector of T (4 floats or 2 doubles): the unit of the physical-field
+// loads and stores, so one instruction moves 16 bytes per lane.
+template <class T>
+struct Vec;
+template <>
+struct Vec<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+    static constexpr int N = 4;
+};
+template <>
+struct Vec<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+    static constexpr int N = 2;
+};
+template <class T>
+using vec_t = typename Vec<T>::type;
+
+// two adjacent complex values as 16-byte accesses (16-byte aligned address)
+template <class T>
+__device__ __forceinline__ void load2(const cx<T> *p, cx<T> &a, cx<T> &b)
+{
+    if constexpr (sizeof(T) == 4) {
+        const vec_t<float> v = *reinterpret_cast<const vec_t<float> *>(p);
+        a = {v[0], v[1]};
+        b = {v[2], v[3]};
+    } else {
+        a = p[0];
+        b = p[1];
+    }
+}
+template <class T>
+__device__ __forceinline__ void store2(cx<T> *p, cx<T> a, cx<T> b)
+{
+    if constexpr (sizeof(T) == 4) {
+        *reinterpret_cast<vec_t<float> *>(p) = vec_t<float>{a.x, a.y, b.x, b.y};
+    } else {
+        p[0] = a;
+        p[1] = b;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // wavenumbers: kx from a table (reference or signed), ky = j*Ky0,
 // operators_par.c:50-71, 105-116, 168-192.
@@ -78,6 +119,21 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
 }
 
 // ---------------------------------------------------------------------------
+// x passes: a workgroup owns ROWS ky-rows of NX (ROWS*NX = max(1024, NX)
+// elements, PER = ROWS*NX/NT of them per thread) and keeps every load of its
+// rows in flight at once (clamped rows, no branches between the loads).
+// Outputs are transformed XNB at a time: XNB*ROWS rows share the stages and
+// barriers of one LDS FFT.
+template <class T, int LOGNX>
+struct XGeom {
+    static constexpr int NX = 1 << LOGNX;
+    static constexpr int ROWS = LOGNX >= 10 ? 1 : (1 << (10 - LOGNX));
+    static constexpr int NE = ROWS * NX;
+    static constexpr int PER = NE / NT;
+    static constexpr int BYTES = NE * (int)sizeof(cx<T>);
+    static constexpr int XNB = BYTES <= 8192 ? 4 : (BYTES <= 16384 ? 2 : 1);
+};
+
 // x inverse pass: spectral (FP64 stage input, with multiplier) -> I.  The
 // source row is read once into registers (FP64) and every output of the
 // group is formed from it.  grid (ceil(Nk/ROWS), ngroups, members)
@@ -86,9 +142,10 @@ __global__ __launch_bounds__(NT) void k_xinv(const cx<double> *__restrict__ Y, l
                                              long sI, long fs, XinvOps ops, KTab kt,
                                              const cx<T> *__restrict__ twx)
 {
-    constexpr int NX = 1 << LOGNX;
-    constexpr int PER = (ROWS * NX + NT - 1) / NT;
-    __shared__ cx<T> buf[ROWS * NX];
+    using G = XGeom<T, LOGNX>;
+    static_assert(G::ROWS == ROWS, "x rows");
+    constexpr int NX = G::NX, NE = G::NE, PER = G::PER, NB = G::XNB;
+    __shared__ cx<T> buf[NB * NE];
     const int grp = blockIdx.y;
     const int ky0 = blockIdx.x * ROWS;
     const cx<double> *src = Y + blockIdx.z * sY + ops.src[grp] * fs;
@@ -98,49 +155,60 @@ __global__ __launch_bounds__(NT) void k_xinv(const cx<double> *__restrict__ Y, l
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int e = threadIdx.x + q * NT;
-        const int r = e / NX, kx = e % NX, ky = ky0 + r;
-        s[q] = {0.0, 0.0};
-        kxv[q] = 0.0;
-        kz[q] = 0.0;
-        th[q] = 1.0;
-        if (e < ROWS * NX && ky < kt.nk) {
-            s[q] = src[(long)ky * NX + kx];
-            kxv[q] = kt.kx[kx];
-            const double kyv = ky * kt.ky0;
-            kz[q] = sqrt(kxv[q] * kxv[q] + kyv * kyv);
-            if (h > 0.0) th[q] = tanh(kz[q] * h);  // d^n/dz^n at finite depth: k^n tanh(kh)^(n odd)
-        }
+        const int kx = e % NX, ky = min(ky0 + e / NX, kt.nk - 1);
+        s[q] = src[(long)ky * NX + kx];
+        kxv[q] = kt.kx[kx];
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = threadIdx.x + q * NT;
+        const int ky = ky0 + e / NX;
+        if (ky >= kt.nk) s[q] = {0.0, 0.0};
+        const double kyv = ky * kt.ky0;
+        kz[q] = sqrt(kxv[q] * kxv[q] + kyv * kyv);
+        th[q] = h > 0.0 ? tanh(kz[q] * h) : 1.0;  // d^n/dz^n at finite depth: k^n tanh(kh)^(n odd)
     }
     cx<T> *base = I + blockIdx.z * sI;
-    for (int o = 0; o < ops.n[grp]; ++o) {
-        const XinvOp op = ops.o[grp][o];
+    const int nops = ops.n[grp];
+    for (int o0 = 0; o0 < nops; o0 += NB) {
+        if (o0) __syncthreads();  // the previous batch's stores have read buf
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int e = threadIdx.x + q * NT;
-            if (e >= ROWS * NX) break;
-            const int ky = ky0 + e / NX;
-            double re = s[q].x, im = s[q].y;
-            if (op.op == OP_DX) {
-                re = -kxv[q] * s[q].y;
-                im = kxv[q] * s[q].x;
-            } else if (op.op == OP_DY) {
-                const double k = ky * kt.ky0;
-                re = -k * s[q].y;
-                im = k * s[q].x;
-            } else if (op.op == OP_DZ) {
-                double m = 1.0;
-                for (int i = 0; i < op.pow; ++i) m *= kz[q];
-                if (op.pow & 1) m *= th[q];
-                re = m * s[q].x;
-                im = m * s[q].y;
+        for (int b = 0; b < NB; ++b) {
+            if (o0 + b >= nops) break;
+            const XinvOp op = ops.o[grp][o0 + b];
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const int e = threadIdx.x + q * NT;
+                const int ky = ky0 + e / NX;
+                double re = s[q].x, im = s[q].y;
+                if (op.op == OP_DX) {
+                    re = -kxv[q] * s[q].y;
+                    im = kxv[q] * s[q].x;
+                } else if (op.op == OP_DY) {
+                    const double k = ky * kt.ky0;
+                    re = -k * s[q].y;
+                    im = k * s[q].x;
+                } else if (op.op == OP_DZ) {
+                    double m = 1.0;
+                    for (int i = 0; i < op.pow; ++i) m *= kz[q];
+                    if (op.pow & 1) m *= th[q];
+                    re = m * s[q].x;
+                    im = m * s[q].y;
+                }
+                buf[b * NE + e] = {T(re), T(im)};
             }
-            buf[e] = {T(re), T(im)};
         }
-        lds_fft<T, LOGNX, ROWS, NT, true>(buf, twx);
-        cx<T> *dst = base + op.dst * fs;
-        for (int e = threadIdx.x; e < ROWS * NX; e += NT) {
-            const int r = e / NX, kx = e % NX, ky = ky0 + r;
-            if (ky < kt.nk) dst[(long)ky * NX + kx] = buf[e];
+        lds_fft<T, LOGNX, ROWS * NB, NT, true>(buf, twx);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            if (o0 + b >= nops) break;
+            cx<T> *dst = base + ops.o[grp][o0 + b].dst * fs;
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const int e = threadIdx.x + q * NT;
+                const int kx = e % NX, ky = ky0 + e / NX;
+                if (ky < kt.nk) dst[(long)ky * NX + kx] = buf[b * NE + e];
+            }
         }
     }
 }
@@ -157,30 +225,54 @@ struct FieldMap {
 // c2r along y of x-rows x0 .. x0+2*NPR-1 of one field: the tile of the
 // x-intermediate src [Nk][nx] is read transposed, rows x0+2p and x0+2p+1 are
 // the real and imaginary part of one complex inverse FFT, written to dst
-// [nx][NY].  buf holds NPR*NY complex values.
+// [nx][NY].  buf holds NPR*NY complex values.  Every load of the tile is
+// issued before the first LDS write; one tile per workgroup.
 template <class T, int LOGNY, int NPR>
 __device__ __forceinline__ void c2r_tile(const cx<T> *__restrict__ src, T *__restrict__ dst, int x0, int nx,
                                          cx<T> *buf, const cx<T> *__restrict__ twy)
 {
-    constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1;
-    __syncthreads();  // buf may still be read by a previous tile's store loop
-    for (int e = threadIdx.x; e < NK * NPR; e += NT) {
+    constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1, NL = NK * NPR, PERL = (NL + NT - 1) / NT;
+    cx<T> A[PERL], B[PERL];
+#pragma unroll
+    for (int q = 0; q < PERL; ++q) {
+        // past the end: reload the last element (in bounds, value unused)
+        const int e = min((int)threadIdx.x + q * NT, NL - 1);
         const int p = e % NPR, ky = e / NPR;
-        cx<T> A = src[(long)ky * nx + x0 + 2 * p];
-        cx<T> B = src[(long)ky * nx + x0 + 2 * p + 1];
-        if (ky == 0 || ky == NY / 2) {
-            A.y = T(0);
-            B.y = T(0);
+        load2(src + (long)ky * nx + x0 + 2 * p, A[q], B[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < PERL; ++q) {
+        const int e = threadIdx.x + q * NT;
+        if ((NL % NT) == 0 || e < NL) {
+            const int p = e % NPR, ky = e / NPR;
+            cx<T> a = A[q], b = B[q];
+            if (ky == 0 || ky == NY / 2) {
+                a.y = T(0);
+                b.y = T(0);
+            }
+            buf[p * NY + ky] = {a.x - b.y, a.y + b.x};
+            if (ky > 0 && ky < NY / 2) buf[p * NY + NY - ky] = {a.x + b.y, b.x - a.y};
         }
-        buf[p * NY + ky] = {A.x - B.y, A.y + B.x};
-        if (ky > 0 && ky < NY / 2) buf[p * NY + NY - ky] = {A.x + B.y, B.x - A.y};
     }
     lds_fft<T, LOGNY, NPR, NT, true>(buf, twy);
-    for (int e = threadIdx.x; e < NPR * NY; e += NT) {
-        const int p = e / NY, y = e % NY;
-        const cx<T> z = buf[e];
-        dst[(long)(x0 + 2 * p) * NY + y] = z.x;
-        dst[(long)(x0 + 2 * p + 1) * NY + y] = z.y;
+    // V consecutive y of both rows per thread: 16-byte stores
+    using F = vec_t<T>;
+    constexpr int V = Vec<T>::N, NYV = NY / V, NS = NPR * NYV, PERS = (NS + NT - 1) / NT;
+#pragma unroll
+    for (int q = 0; q < PERS; ++q) {
+        const int e = threadIdx.x + q * NT;
+        if ((NS % NT) == 0 || e < NS) {
+            const int p = e / NYV, y = (e % NYV) * V;
+            F ra, rb;
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const cx<T> z = buf[p * NY + y + v];
+                ra[v] = z.x;
+                rb[v] = z.y;
+            }
+            *reinterpret_cast<F *>(dst + (long)(x0 + 2 * p) * NY + y) = ra;
+            *reinterpret_cast<F *>(dst + (long)(x0 + 2 * p + 1) * NY + y) = rb;
+        }
     }
 }
 
@@ -207,6 +299,29 @@ struct Phys {
     {
         return P[m * sP + f * fsP + (long)x * ny + y];
     }
+    // y .. y+V-1 (y a multiple of V)
+    __device__ __forceinline__ vec_t<T> vat(int m, int f, int x, int y) const
+    {
+        return *reinterpret_cast<const vec_t<T> *>(P + m * sP + f * fsP + (long)x * ny + y);
+    }
+};
+
+// Kahan-compensated (COMP) or plain sum, lane-wise over a vector type
+template <class F, bool COMP>
+struct KSum {
+    F s = F(0), c = F(0);
+    __device__ __forceinline__ void add(F v)
+    {
+        if constexpr (COMP) {
+            const F y = v - c;
+            const F t = s + y;
+            c = (t - s) - y;
+            s = t;
+        } else {
+            s += v;
+        }
+    }
+    __device__ __forceinline__ F value() const { return s - c; }
 };
 
 // field indices of the physical fields of one member
@@ -237,35 +352,7 @@ struct Sum {
     __device__ __forceinline__ T value() const { return s - c; }
 };
 
-// plain read of one field
-template <class T>
-struct ProRead {
-    Phys<T> ph;
-    int f;
-    __device__ __forceinline__ T operator()(int m, int x, int y) const { return ph.at(m, f, x, y); }
-};
-
-// phi^(lev+1) = - sum_{k=1}^{lev} eta^k/k! d^k phi^(lev+1-k)/dz^k
-// (West et al. 1987; the reference's hphin update, hosm_2d_par.c:302-317)
-template <class T>
-struct ProPhi {
-    Phys<T> ph;
-    HosIdx ix;
-    int lev;
-    bool comp;
-    __device__ __forceinline__ T operator()(int m, int x, int y) const
-    {
-        const T e = ph.at(m, ix.eta, x, y);
-        T ek = T(1);
-        Sum<T> s(comp);
-        for (int k = 1; k <= lev; ++k) {
-            ek *= e;
-            s.add(ek * T(ix.invfact[k]) * ph.at(m, ix.d[lev + 1 - k][k], x, y));
-        }
-        return -s.value();
-    }
-};
-
+// scalar order sums (the run-time-order Zvel prologue)
 // W^(a) = sum_{k=0}^{a-1} eta^k/k! d^{k+1} phi^(a-k)/dz^{k+1}, a = 1..M
 // (the reference's hwn, hosm_2d_par.c:322-357)
 template <class T>
@@ -302,41 +389,153 @@ __device__ __forceinline__ T w_upto(const T *W, int a0, int K, bool comp)
     return s.value();
 }
 
+// Prologues of the y r2c pass.  A prologue computes V = Vec<T>::N
+// consecutive y values of one x-row: single-output ones return them, pair
+// ones write two vectors.  The order-templated prologues issue every field
+// load of the V points before the first use.
+
+// plain read of one field
+template <class T>
+struct ProRead {
+    Phys<T> ph;
+    int f;
+    __device__ __forceinline__ vec_t<T> operator()(int m, int x, int y) const { return ph.vat(m, f, x, y); }
+};
+
+// phi^(LEV+1) = - sum_{k=1}^{LEV} eta^k/k! d^k phi^(LEV+1-k)/dz^k
+// (West et al. 1987; the reference's hphin update, hosm_2d_par.c:302-317)
+template <class T, int LEV>
+struct ProPhi {
+    Phys<T> ph;
+    HosIdx ix;
+    bool comp;
+    using F = vec_t<T>;
+    template <bool C>
+    __device__ __forceinline__ F calc(const F *f) const
+    {
+        const F e = f[0];
+        F ek = F(1);
+        KSum<F, C> s;
+#pragma unroll
+        for (int k = 1; k <= LEV; ++k) {
+            ek *= e;
+            s.add(ek * T(ix.invfact[k]) * f[k]);
+        }
+        return -s.value();
+    }
+    __device__ __forceinline__ F operator()(int m, int x, int y) const
+    {
+        F f[LEV + 1];
+        f[0] = ph.vat(m, ix.eta, x, y);
+#pragma unroll
+        for (int k = 1; k <= LEV; ++k) f[k] = ph.vat(m, ix.d[LEV + 1 - k][k], x, y);
+        return comp ? calc<true>(f) : calc<false>(f);
+    }
+};
+
+// position of d^j phi^(a)/dz^j among the M(M+1)/2 fields a = 1..M, j = 1..M-a+1
+template <int M>
+__host__ __device__ constexpr int d_pos(int a, int j)
+{
+    int p = 0;
+    for (int b = 1; b < a; ++b) p += M - b + 1;
+    return p + j - 1;
+}
+
+// W^(a) = sum_{k=0}^{a-1} eta^k/k! d^{k+1} phi^(a-k)/dz^{k+1}, a = 1..M
+// (the reference's hwn, hosm_2d_par.c:322-357); d holds the fields in d_pos order
+template <class F, class T, int M, bool C>
+__device__ __forceinline__ void w_orders_v(const HosIdx &ix, F e, const F *d, F *W)
+{
+#pragma unroll
+    for (int a = 1; a <= M; ++a) {
+        F ek = F(1);
+        KSum<F, C> s;
+#pragma unroll
+        for (int k = 0; k < a; ++k) {
+            s.add(ek * T(ix.invfact[k]) * d[d_pos<M>(a - k, k + 1)]);
+            ek *= e;
+        }
+        W[a] = s.value();
+    }
+}
+
+// (W^2)^(K) = sum_{a,b>=1, a+b<=K} W^(a) W^(b)  (hosm_2d_par.c:397-442)
+template <class F, int K, bool C>
+__device__ __forceinline__ F w2_upto_v(const F *W)
+{
+    KSum<F, C> s;
+#pragma unroll
+    for (int a = 1; a < K; ++a)
+#pragma unroll
+        for (int b = 1; a + b <= K; ++b) s.add(W[a] * W[b]);
+    return s.value();
+}
+
+// sum_{a=A0}^{K} W^(a), small terms first
+template <class F, int A0, int K, bool C>
+__device__ __forceinline__ F w_upto_v(const F *W)
+{
+    KSum<F, C> s;
+#pragma unroll
+    for (int a = K; a >= A0; --a) s.add(W[a]);
+    return s.value();
+}
+
 // nonlinear RHS in physical space (rhs_hos, hosm_2d_par.c:149-206) without
 // the two linear terms W^(1) and -g eta, which the x epilogue adds in FP64:
 //   N_eta = -(eta_x phi_x + eta_y phi_y) + (W^(M) - W^(1)) + |grad eta|^2 W^(M-2)
 //   N_phi = 1/2[(W^2)^(M) + |grad eta|^2 (W^2)^(M-2)] - 1/2 |grad phi|^2
-template <class T>
+template <class T, int M>
 struct ProRhs {
     Phys<T> ph;
     HosIdx ix;
     bool comp;
-    __device__ __forceinline__ void operator()(int m, int x, int y, T &a, T &b) const
+    using F = vec_t<T>;
+    static constexpr int ND = M * (M + 1) / 2;
+    template <bool C>
+    __device__ __forceinline__ void calc(const F *f, F &a, F &b) const
     {
-        const int M = ix.M;
-        const T e = ph.at(m, ix.eta, x, y);
-        const T ex = ph.at(m, ix.etax, x, y), ey = ph.at(m, ix.etay, x, y);
-        const T px = ph.at(m, ix.phix, x, y), py = ph.at(m, ix.phiy, x, y);
-        T W[MAXM + 1];
-        w_orders<T>(ph, ix, m, x, y, e, W, comp);
-        const T G = ex * ex + ey * ey;
-        const T wM2 = (M - 2 >= 1) ? w_upto<T>(W, 1, M - 2, comp) : T(0);
-        const T wMnl = w_upto<T>(W, 2, M, comp);
+        const F e = f[0], ex = f[1], ey = f[2], px = f[3], py = f[4];
+        F W[M + 1];
+        w_orders_v<F, T, M, C>(ix, e, f + 5, W);
+        const F G = ex * ex + ey * ey;
+        F wM2 = F(0), w2M2 = F(0);
+        if constexpr (M - 2 >= 1) wM2 = w_upto_v<F, 1, M - 2, C>(W);
+        const F wMnl = w_upto_v<F, 2, M, C>(W);
         a = -(ex * px + ey * py) + wMnl + G * wM2;
-        const T w2M = w2_upto<T>(W, M, comp);
-        const T w2M2 = w2_upto<T>(W, M - 2, comp);
+        const F w2M = w2_upto_v<F, M, C>(W);
+        if constexpr (M - 2 >= 1) w2M2 = w2_upto_v<F, M - 2, C>(W);
         b = T(0.5) * (w2M + G * w2M2) - T(0.5) * (px * px + py * py);
+    }
+    __device__ __forceinline__ void operator()(int m, int x, int y, F &a, F &b) const
+    {
+        F f[5 + ND];
+        f[0] = ph.vat(m, ix.eta, x, y);
+        f[1] = ph.vat(m, ix.etax, x, y);
+        f[2] = ph.vat(m, ix.etay, x, y);
+        f[3] = ph.vat(m, ix.phix, x, y);
+        f[4] = ph.vat(m, ix.phiy, x, y);
+#pragma unroll
+        for (int p = 1; p <= M; ++p)
+#pragma unroll
+            for (int j = 1; j <= M - p + 1; ++j) f[5 + d_pos<M>(p, j)] = ph.vat(m, ix.d[p][j], x, y);
+        if (comp)
+            calc<true>(f, a, b);
+        else
+            calc<false>(f, a, b);
     }
 };
 
-// Zvel outputs: which=0 -> (W^(M), W^(M-2)), which=1 -> ((W^2)^(M), (W^2)^(M-2))
+// Zvel outputs: which=0 -> (W^(M), W^(M-2)), which=1 -> ((W^2)^(M), (W^2)^(M-2)),
+// order M at run time, one point at a time (the Zvel entry point, not the step)
 template <class T>
 struct ProZvel {
     Phys<T> ph;
     HosIdx ix;
     int which;
     bool comp;
-    __device__ __forceinline__ void operator()(int m, int x, int y, T &a, T &b) const
+    __device__ __forceinline__ void point(int m, int x, int y, T &a, T &b) const
     {
         const int M = ix.M;
         const T e = ph.at(m, ix.eta, x, y);
@@ -350,40 +549,51 @@ struct ProZvel {
             b = w2_upto<T>(W, M - 2, comp);
         }
     }
+    __device__ __forceinline__ void operator()(int m, int x, int y, vec_t<T> &a, vec_t<T> &b) const
+    {
+#pragma unroll
+        for (int v = 0; v < Vec<T>::N; ++v) {
+            T sa, sb;
+            point(m, x, y + v, sa, sb);
+            a[v] = sa;
+            b[v] = sb;
+        }
+    }
 };
-
-// ---------------------------------------------------------------------------
 // r2c along y of x-rows x0 .. x0+2*NPR-1 of the field the prologue computes
 // on the fly: rows x0+2p and x0+2p+1 are the real and imaginary part of one
 // complex FFT; the half spectra are written transposed into out [Nk][nx].
+// One tile per workgroup.
 template <class T, int LOGNY, int NPR, class PRO>
 __device__ __forceinline__ void r2c_tile(const PRO &pro, int m, cx<T> *__restrict__ out, int x0, int nx,
                                          cx<T> *buf, const cx<T> *__restrict__ twy)
 {
-    constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1, TX = 2 * NPR;
-    __syncthreads();
-    // compile-time trip count, unrolled: the loads of several points are in
-    // flight together (the prologue reads up to M+1 fields per point)
-    constexpr int PER = (NPR * NY + NT - 1) / NT;
+    using F = vec_t<T>;
+    constexpr int V = Vec<T>::N;
+    constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1, NYV = NY / V, NTASK = NPR * NYV;
+    constexpr int PER = (NTASK + NT - 1) / NT;
+    // one task = V consecutive y of the row pair; the loop is unrolled so the
+    // loads of several tasks are in flight together
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int e = threadIdx.x + q * NT;
-        if (e < NPR * NY) {
-            const int p = e / NY, y = e % NY;
-            buf[e] = {pro(m, x0 + 2 * p, y), pro(m, x0 + 2 * p + 1, y)};
+        if ((NTASK % NT) == 0 || e < NTASK) {
+            const int p = e / NYV, y = (e % NYV) * V;
+            const F ra = pro(m, x0 + 2 * p, y);
+            const F rb = pro(m, x0 + 2 * p + 1, y);
+            cx<T> *d = buf + p * NY + y;
+#pragma unroll
+            for (int v = 0; v < V; ++v) d[v] = {ra[v], rb[v]};
         }
     }
     lds_fft<T, LOGNY, NPR, NT, false>(buf, twy);
-    for (int e = threadIdx.x; e < NK * TX; e += NT) {
-        const int t = e % TX, ky = e / TX, p = t >> 1;
+    // the two real rows of pair p at ky from Z(ky) and Z(NY-ky)
+    for (int e = threadIdx.x; e < NK * NPR; e += NT) {
+        const int p = e % NPR, ky = e / NPR;
         const cx<T> zk = buf[p * NY + ky];
         const cx<T> zm = buf[p * NY + ((NY - ky) & (NY - 1))];
-        cx<T> v;
-        if ((t & 1) == 0)
-            v = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
-        else
-            v = {T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x)};
-        out[(long)ky * nx + x0 + t] = v;
+        store2(out + (long)ky * nx + x0 + 2 * p, cx<T>{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)},
+               cx<T>{T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x)});
     }
 }
 
@@ -394,29 +604,29 @@ __device__ __forceinline__ void r2c_pair_tile(const PRO &pro, int m, cx<T> *__re
                                               cx<T> *__restrict__ ob, int x0, int nx, cx<T> *buf,
                                               const cx<T> *__restrict__ twy)
 {
-    constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1;
-    __syncthreads();
-    constexpr int PER = (NROW * NY + NT - 1) / NT;
-#pragma unroll 2
+    using F = vec_t<T>;
+    constexpr int V = Vec<T>::N;
+    constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1, NYV = NY / V, NTASK = NROW * NYV;
+    constexpr int PER = (NTASK + NT - 1) / NT;
+#pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int e = threadIdx.x + q * NT;
-        if (e < NROW * NY) {
-            const int p = e / NY, y = e % NY;
-            T a, b;
+        if ((NTASK % NT) == 0 || e < NTASK) {
+            const int p = e / NYV, y = (e % NYV) * V;
+            F a, b;
             pro(m, x0 + p, y, a, b);
-            buf[e] = {a, b};
+            cx<T> *d = buf + p * NY + y;
+#pragma unroll
+            for (int v = 0; v < V; ++v) d[v] = {a[v], b[v]};
         }
     }
     lds_fft<T, LOGNY, NROW, NT, false>(buf, twy);
-    for (int e = threadIdx.x; e < NK * NROW * 2; e += NT) {
-        const int t = e % (2 * NROW), ky = e / (2 * NROW);
-        const int p = t % NROW, which = t / NROW;
+    for (int e = threadIdx.x; e < NK * NROW; e += NT) {
+        const int p = e % NROW, ky = e / NROW;
         const cx<T> zk = buf[p * NY + ky];
         const cx<T> zm = buf[p * NY + ((NY - ky) & (NY - 1))];
-        if (which == 0)
-            oa[(long)ky * nx + x0 + p] = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
-        else
-            ob[(long)ky * nx + x0 + p] = {T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x)};
+        oa[(long)ky * nx + x0 + p] = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
+        ob[(long)ky * nx + x0 + p] = {T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x)};
     }
 }
 
@@ -446,53 +656,69 @@ __global__ __launch_bounds__(NT) void k_yr2c_pair(PRO pro, cx<T> *__restrict__ I
 
 // ---------------------------------------------------------------------------
 // x forward pass of phi^(lev+1), then d^j/dz^j for j = 1..ndst and the x
-// inverse of each: the spectral phi^(lev+1) never leaves LDS.
+// inverse of each: the spectral phi^(lev+1) never leaves the chip (each
+// thread keeps its modes in registers; the inverses run XNB rows at a time).
 // grid (ceil(Nk/ROWS), 1, members)
 template <class T, int LOGNX, int ROWS>
 __global__ __launch_bounds__(NT) void k_xfwdinv(cx<T> *__restrict__ I, long sI, long fs, int src, int dst0, int ndst,
                                                 double scale, KTab kt, const cx<T> *__restrict__ twx)
 {
-    constexpr int NX = 1 << LOGNX;
-    constexpr int PER = (ROWS * NX + NT - 1) / NT;
-    __shared__ cx<T> S[ROWS * NX];
-    __shared__ cx<T> W[ROWS * NX];
+    using G = XGeom<T, LOGNX>;
+    static_assert(G::ROWS == ROWS, "x rows");
+    constexpr int NX = G::NX, NE = G::NE, PER = G::PER, NB = G::XNB;
+    __shared__ cx<T> W[NB * NE];
     const int ky0 = blockIdx.x * ROWS;
     cx<T> *base = I + blockIdx.z * sI;
     const cx<T> *in = base + src * fs;
-    for (int e = threadIdx.x; e < ROWS * NX; e += NT) {
-        const int r = e / NX, kx = e % NX, ky = ky0 + r;
-        S[e] = (ky < kt.nk) ? in[(long)ky * NX + kx] : cx<T>{T(0), T(0)};
+    cx<T> s[PER];
+    double kxv[PER], kz[PER], th[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = threadIdx.x + q * NT;
+        const int kx = e % NX, ky = min(ky0 + e / NX, kt.nk - 1);
+        s[q] = in[(long)ky * NX + kx];
+        kxv[q] = kt.kx[kx];
     }
-    // |k| and tanh(|k| h) of this thread's modes, FP64
-    double kz[PER], th[PER];
     const double h = kt.h(blockIdx.z);
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int e = threadIdx.x + q * NT;
-        const int kx = e % NX, ky = ky0 + e / NX;
-        const double kxv = kt.kx[kx], kyv = ky * kt.ky0;
-        kz[q] = sqrt(kxv * kxv + kyv * kyv);
+        const int ky = ky0 + e / NX;
+        W[e] = (ky < kt.nk) ? s[q] : cx<T>{T(0), T(0)};
+        const double kyv = ky * kt.ky0;
+        kz[q] = sqrt(kxv[q] * kxv[q] + kyv * kyv);  // |k| and tanh(|k| h), FP64
         th[q] = h > 0.0 ? tanh(kz[q] * h) : 1.0;
     }
-    lds_fft<T, LOGNX, ROWS, NT, false>(S, twx);
+    lds_fft<T, LOGNX, ROWS, NT, false>(W, twx);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) s[q] = W[threadIdx.x + q * NT];
     double kj[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) kj[q] = scale;
-    for (int j = 1; j <= ndst; ++j) {
+    for (int j0 = 1; j0 <= ndst; j0 += NB) {
+        __syncthreads();  // W has been read (forward result or the previous batch)
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int e = threadIdx.x + q * NT;
-            if (e >= ROWS * NX) break;
-            kj[q] *= kz[q];  // scale * |k|^j
-            const T mul = T((j & 1) ? kj[q] * th[q] : kj[q]);
-            const cx<T> s = S[e];
-            W[e] = {mul * s.x, mul * s.y};
+        for (int b = 0; b < NB; ++b) {
+            const int j = j0 + b;
+            if (j > ndst) break;
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                kj[q] *= kz[q];  // scale * |k|^j
+                const T mul = T((j & 1) ? kj[q] * th[q] : kj[q]);
+                W[b * NE + threadIdx.x + q * NT] = {mul * s[q].x, mul * s[q].y};
+            }
         }
-        lds_fft<T, LOGNX, ROWS, NT, true>(W, twx);
-        cx<T> *out = base + (dst0 + j - 1) * fs;
-        for (int e = threadIdx.x; e < ROWS * NX; e += NT) {
-            const int r = e / NX, kx = e % NX, ky = ky0 + r;
-            if (ky < kt.nk) out[(long)ky * NX + kx] = W[e];
+        lds_fft<T, LOGNX, ROWS * NB, NT, true>(W, twx);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            if (j0 + b > ndst) break;
+            cx<T> *out = base + (dst0 + j0 + b - 1) * fs;
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const int e = threadIdx.x + q * NT;
+                const int kx = e % NX, ky = ky0 + e / NX;
+                if (ky < kt.nk) out[(long)ky * NX + kx] = W[b * NE + e];
+            }
         }
     }
 }
@@ -527,18 +753,31 @@ __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, lo
                                                  const cx<double> *Y, long sY, double scale, KTab kt,
                                                  RkArgs rk, const cx<T> *__restrict__ twx)
 {
-    constexpr int NX = 1 << LOGNX;
+    using G = XGeom<T, LOGNX>;
+    static_assert(G::ROWS == ROWS, "x rows");
+    constexpr int NX = G::NX, PER = G::PER;
     __shared__ cx<T> S[2 * ROWS * NX];
     __shared__ double red[NT / 64];
     const int ky0 = blockIdx.x * ROWS;
     const int m = blockIdx.z;
     const cx<T> *ia = I + m * sI + iA * fs;
     const cx<T> *ib = I + m * sI + iB * fs;
-    for (int e = threadIdx.x; e < ROWS * NX; e += NT) {
-        const int r = e / NX, kx = e % NX, ky = ky0 + r;
-        const bool ok = ky < kt.nk;
-        S[e] = ok ? ia[(long)ky * NX + kx] : cx<T>{T(0), T(0)};
-        S[ROWS * NX + e] = ok ? ib[(long)ky * NX + kx] : cx<T>{T(0), T(0)};
+    {
+        cx<T> va[PER], vb[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = threadIdx.x + q * NT;
+            const long idx = (long)min(ky0 + e / NX, kt.nk - 1) * NX + e % NX;
+            va[q] = ia[idx];
+            vb[q] = ib[idx];
+        }
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = threadIdx.x + q * NT;
+            const bool ok = ky0 + e / NX < kt.nk;
+            S[e] = ok ? va[q] : cx<T>{T(0), T(0)};
+            S[ROWS * NX + e] = ok ? vb[q] : cx<T>{T(0), T(0)};
+        }
     }
     lds_fft<T, LOGNX, 2 * ROWS, NT, false>(S, twx);
 
@@ -551,7 +790,9 @@ __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, lo
         R = 1.0 - exp(-tt * tt / rk.tramp / rk.tramp);  // RampFun, hosm_2d_par.c:526-530
     }
     const double dt = (rk.mode == EPI_RK_SUBDIAG) ? rk.dt[m] : 0.0;
-    for (int e = threadIdx.x; e < ROWS * NX; e += NT) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = threadIdx.x + q * NT;
         const int r = e / NX, kx = e % NX, ky = ky0 + r;
         if (ky >= kt.nk) continue;
         const long idx = (long)ky * NX + kx;
@@ -738,9 +979,9 @@ template <class T>
 struct ProMul {
     Phys<T> ph;
     int fa, fb;
-    __device__ __forceinline__ T operator()(int m, int x, int y) const
+    __device__ __forceinline__ vec_t<T> operator()(int m, int x, int y) const
     {
-        return ph.at(m, fa, x, y) * ph.at(m, fb, x, y);
+        return ph.vat(m, fa, x, y) * ph.vat(m, fb, x, y);
     }
 };
 
@@ -749,7 +990,22 @@ struct ProMul {
 __global__ __launch_bounds__(NT) void k_copy_c64(const cx<double> *__restrict__ src, cx<double> *__restrict__ dst,
                                                  long n)
 {
-    for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) dst[i] = src[i];
+    // four 16-byte loads in flight per thread per iteration
+    constexpr int U = 4;
+    const long step = (long)gridDim.x * NT * U;
+    for (long i0 = blockIdx.x * (long)NT * U + threadIdx.x; i0 < n; i0 += step) {
+        cx<double> v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long i = i0 + (long)u * NT;
+            if (i < n) v[u] = src[i];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long i = i0 + (long)u * NT;
+            if (i < n) dst[i] = v[u];
+        }
+    }
 }
 
 // real [Nx][Ny] (T) <-> padded double host layout helpers
