@@ -63,7 +63,7 @@ class Jonswap(ctypes.Structure):
 
 def build() -> str:
     """Compile libhos_gpu.so for gfx950 (hos_amd/Makefile)."""
-    subprocess.run(["make", "-s", "-C", _HERE, "-B", "libhos_gpu.so"], check=True)
+    subprocess.run(["make", "-s", "-j8", "-C", _HERE, "-B", "all"], check=True)
     return LIB_PATH
 
 

@@ -1,0 +1,346 @@
+// hos_ctx.hpp -- the solver context and the kernel launchers (Eng<T>),
+// shared by hos_engine.hip (host orchestration, C ABI) and hos_levels.hip
+// (the order-templated y r2c launches, built in parallel units).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "../../include/hos_gpu.h"
+#include "hos_kernels.hpp"
+
+using namespace hos;
+
+// records the message for hos_last_error() and returns code (hos_engine.hip)
+__attribute__((visibility("hidden"))) int fail(int code, const char *fmt, ...);
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) return fail(HOS_E_DEVICE, "%s: %s", #expr, hipGetErrorString(e_));   \
+    } while (0)
+
+
+constexpr int LOGN_MIN = 4, LOGN_MAX = 11;
+
+struct hos_ctx {
+    hos_params p{};
+    int nm = 0, M = 0, L = 0, Nk = 0, lognx = 0, logny = 0, mx = 0, my = 0;
+    int nI = 0, nP = 0;
+    int tx[3] = {8, 4, 4};  // y tiles: c2r, r2c, pair
+    bool f32 = false;
+    long fs = 0;    // complex elements per spectral field  (Nk*Nx)
+    long fsP = 0;   // real elements per physical field     (Nx*Ny)
+    hipStream_t stream = nullptr;
+    // device memory
+    cx<double> *U = nullptr, *A = nullptr, *Y = nullptr, *K = nullptr;
+    void *I = nullptr, *P = nullptr;
+    void *twx = nullptr, *twy = nullptr;
+    double *kx = nullptr, *t = nullptr, *dt = nullptr, *H = nullptr, *vol = nullptr, *hpart = nullptr,
+           *mean0 = nullptr, *depth = nullptr;
+    double *th = nullptr;  // [nm][Nk][Nx] tanh(|k| h) per member
+    int active = 0;  // members 0..active-1 are stepped
+    std::vector<hipGraphExec_t> gexec_by_active;
+    std::vector<hipGraph_t> graph_by_active;
+    bool async_pending = false;
+    int async_steps = 0;
+    double *hostbuf = nullptr;  // device staging [Nx][Ny] double
+    int nK = 0;                 // number of FP64 [2][Nk][Nx] slots in K (per member for RK)
+    HosIdx ix{};
+    // Butcher table, time_schemes_par.c:127-314
+    int S = 0;
+    double a[7] = {}, b[7][7] = {}, c[7] = {};
+    bool subdiag = true;
+    // timing / graph
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    int use_graph = 0;
+    std::vector<double> host_t;   // host mirror of t
+    std::vector<double> host_dt;  // host mirror of dt
+    // per-step and per-kernel HIP-event timing
+    std::vector<hipEvent_t> step_ev;
+    std::vector<double> step_ms;
+    int prof_on = 0;
+    bool capturing = false;
+    struct ProfRec {
+        int kind;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    std::vector<ProfRec> recs;
+    std::vector<hipEvent_t> evpool;
+    size_t evused = 0;
+    double kms[HOS_K_NKINDS] = {}, kbytes[HOS_K_NKINDS] = {};
+    long kn[HOS_K_NKINDS] = {};
+};
+
+inline hipEvent_t pool_event(hos_ctx *c)
+{
+    if (c->evused < c->evpool.size()) return c->evpool[c->evused++];
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    c->evpool.push_back(e);
+    c->evused++;
+    return e;
+}
+
+// Records a HIP event pair around one kernel launch on the context's stream
+// when kernel profiling is on (eager launches only).
+struct Prof {
+    hos_ctx *c;
+    int kind;
+    double bytes;
+    hipEvent_t a = nullptr;
+    Prof(hos_ctx *c_, int k, double by) : c(c_), kind(k), bytes(by)
+    {
+        if (c->prof_on && !c->capturing) {
+            a = pool_event(c);
+            (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~Prof()
+    {
+        if (a) {
+            hipEvent_t b = pool_event(c);
+            (void)hipEventRecord(b, c->stream);
+            c->recs.push_back({kind, a, b, bytes});
+        }
+    }
+};
+
+// kernel dispatch on (T, log2 N, tile)
+inline KTab ktab(const hos_ctx *c)
+{
+    KTab k;
+    k.kx = c->kx;
+    k.ky0 = 2 * M_PI / c->p.Ly;
+    k.depth = c->depth;
+    k.th = c->th;
+    k.ths = c->fs;
+    k.nk = c->Nk;
+    k.mx = c->mx;
+    k.my = c->my;
+    return k;
+}
+
+#define LOGN_SWITCH(v, ...)                                                                       \
+    switch (v) {                                                                                   \
+    case 4: { constexpr int LG = 4; __VA_ARGS__; } break;                                                 \
+    case 5: { constexpr int LG = 5; __VA_ARGS__; } break;                                                 \
+    case 6: { constexpr int LG = 6; __VA_ARGS__; } break;                                                 \
+    case 7: { constexpr int LG = 7; __VA_ARGS__; } break;                                                 \
+    case 8: { constexpr int LG = 8; __VA_ARGS__; } break;                                                 \
+    case 9: { constexpr int LG = 9; __VA_ARGS__; } break;                                                 \
+    case 10: { constexpr int LG = 10; __VA_ARGS__; } break;                                               \
+    case 11: { constexpr int LG = 11; __VA_ARGS__; } break;                                               \
+    default: return fail(HOS_E_INVALID, "unsupported size 2^%d", v);                                \
+    }
+
+// TXV = y tile; only tiles whose LDS fits 64 KiB are instantiated
+#define TX_CASE(T, V, ...)                                                                         \
+    case V: {                                                                                      \
+        constexpr int TXV = V;                                                                     \
+        if constexpr ((TXV / 2) * (1 << LG) * sizeof(cx<T>) <= 65536) {                            \
+            __VA_ARGS__;                                                                           \
+        } else {                                                                                   \
+            return fail(HOS_E_INVALID, "tile %d exceeds 64 KiB of LDS at Ny = %d", V, 1 << LG);     \
+        }                                                                                          \
+    } break;
+#define TX_SWITCH(T, tx, ...)                                                                      \
+    switch (tx) {                                                                                  \
+        TX_CASE(T, 4, __VA_ARGS__)                                                                 \
+        TX_CASE(T, 8, __VA_ARGS__)                                                                 \
+        TX_CASE(T, 16, __VA_ARGS__)                                                                \
+    default: return fail(HOS_E_INVALID, "unsupported tile %d", tx);                                \
+    }
+
+template <class T>
+struct Eng {
+    hos_ctx *c;
+    int nm;  // members in this launch (from member 0)
+    cx<T> *I() const { return (cx<T> *)c->I; }
+    T *P() const { return (T *)c->P; }
+    long sI() const { return (long)c->nI * c->fs; }
+    long sP() const { return (long)c->nP * c->fsP; }
+    long sY() const { return 2 * c->fs; }
+    const cx<T> *twx() const { return (const cx<T> *)c->twx; }
+    const cx<T> *twy() const { return (const cx<T> *)c->twy; }
+    Phys<T> phys() const { return Phys<T>{P(), sP(), c->fsP, c->p.Ny}; }
+    bool comp() const { return c->f32 && c->p.compensated; }
+
+    // outputs list[i] (dst = intermediate index), grouped by source into
+    // chunks of at most `per` outputs, one chunk per blockIdx.y.  Few rows in
+    // flight (one member): small chunks for more workgroups; many members:
+    // one chunk per source, so each source row is read once.
+    int xinv(const cx<double> *Y, const std::vector<XinvOp> &list, int per = 0)
+    {
+        if (per <= 0) per = ((long)nm * c->Nk >= 4096) ? MAXPERGRP : 3;
+        XinvOps ops{};
+        for (int s = 0; s < 2; ++s) {
+            int k = 0;
+            for (const XinvOp &o : list) {
+                if (o.src != s) continue;
+                if (k == 0) {
+                    if (ops.ngroups == MAXGRP) return fail(HOS_E_INVALID, "too many x-inverse outputs");
+                    ops.src[ops.ngroups] = s;
+                    ops.n[ops.ngroups] = 0;
+                    ops.ngroups++;
+                }
+                const int g = ops.ngroups - 1;
+                ops.o[g][ops.n[g]++] = o;
+                k = (k + 1) % std::min(per, MAXPERGRP);
+            }
+        }
+        const int ngroups = ops.ngroups;
+        const int nops = (int)list.size();
+        LOGN_SWITCH(c->lognx, {
+            constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
+            dim3 grid((c->Nk + ROWS - 1) / ROWS, ngroups, nm);
+            // FP64 form: the tanh(|k| h) table is read once per group
+            const double thb = sizeof(T) == 8 ? 8.0 * ngroups : 0.0;
+            Prof pf(c, HOS_K_XINV, (double)nm * c->fs * (16.0 * ngroups + thb + 2.0 * sizeof(T) * nops));
+            hipLaunchKernelGGL((k_xinv<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, Y, sY(), I(), sI(), c->fs, ops,
+                               ktab(c), twx());
+        });
+        return HOS_OK;
+    }
+    int yc2r(const FieldMap &fm, int nf)
+    {
+        const int tx = c->tx[HOS_TILE_C2R];
+        LOGN_SWITCH(c->logny, {
+            TX_SWITCH(T, tx, {
+                dim3 grid(c->p.Nx / TXV, nf, nm);
+                Prof pf(c, HOS_K_YC2R, (double)nm * nf * sizeof(T) * (2.0 * c->fs + c->fsP));
+                hipLaunchKernelGGL((k_yc2r<T, LG, TXV>), grid, dim3(NT), 0, c->stream, I(), sI(), c->fs, P(), sP(),
+                                   c->fsP, fm, c->p.Nx, twy());
+            })
+        });
+        return HOS_OK;
+    }
+    template <class PRO>
+    int yr2c(const PRO &pro, int dst, int nin)
+    {
+        const int tx = c->tx[HOS_TILE_R2C];
+        LOGN_SWITCH(c->logny, {
+            TX_SWITCH(T, tx, {
+                dim3 grid(c->p.Nx / TXV, 1, nm);
+                Prof pf(c, HOS_K_YR2C, (double)nm * sizeof(T) * (nin * (double)c->fsP + 2.0 * c->fs));
+                hipLaunchKernelGGL((k_yr2c<T, LG, TXV, PRO>), grid, dim3(NT), 0, c->stream, pro, I(), sI(), c->fs,
+                                   dst, c->p.Nx, twy());
+            })
+        });
+        return HOS_OK;
+    }
+    template <class PRO>
+    int yr2c_pair(const PRO &pro, int da, int db, int nin)
+    {
+        const int tx = c->tx[HOS_TILE_PAIR];
+        LOGN_SWITCH(c->logny, {
+            TX_SWITCH(T, tx, {
+                constexpr int TXP = TXV / 2;
+                dim3 grid(c->p.Nx / TXP, 1, nm);
+                Prof pf(c, HOS_K_YR2C_PAIR, (double)nm * sizeof(T) * (nin * (double)c->fsP + 4.0 * c->fs));
+                hipLaunchKernelGGL((k_yr2c_pair<T, LG, TXP, PRO>), grid, dim3(NT), 0, c->stream, pro, I(), sI(),
+                                   c->fs, da, db, c->p.Nx, twy());
+            })
+        });
+        return HOS_OK;
+    }
+    int xfwdinv(int src, int dst0, int ndst)
+    {
+        const double scale = 1.0 / ((double)c->p.Nx * c->p.Ny);
+        LOGN_SWITCH(c->lognx, {
+            constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
+            dim3 grid((c->Nk + ROWS - 1) / ROWS, 1, nm);
+            Prof pf(c, HOS_K_XFWDINV, (double)nm * c->fs * (2.0 * sizeof(T) * (1 + ndst) + (sizeof(T) == 8 ? 8.0 : 0.0)));
+            hipLaunchKernelGGL((k_xfwdinv<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, I(), sI(), c->fs, src, dst0,
+                               ndst, scale, ktab(c), twx());
+        });
+        return HOS_OK;
+    }
+    int xfwd_rhs(int ia, int ib, const cx<double> *Y, const RkArgs &rk)
+    {
+        const double scale = 1.0 / ((double)c->p.Nx * c->p.Ny);
+        // FP64 [2][Nk][Nx] fields moved by the epilogue
+        double f64 = 0.0;
+        if (rk.mode == EPI_RK_SUBDIAG)
+            f64 = 2 /*Y*/ + 2 /*u*/ + (rk.stage > 0 ? 2 : 0) /*acc*/ + 2 /*acc or u*/ + 2 /*Y next*/;
+        else if (rk.mode == EPI_STORE_K)
+            f64 = 2 + 2 + (rk.hpart && rk.stage == 0 ? 2 : 0);
+        else
+            f64 = 2;
+        if (rk.mode != EPI_STORE_SPEC2) f64 += 0.5;  // tanh(|k| h) table
+        LOGN_SWITCH(c->lognx, {
+            constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
+            dim3 grid((c->Nk + ROWS - 1) / ROWS, 1, nm);
+            Prof pf(c, HOS_K_XFWD_RHS, (double)nm * c->fs * (2.0 * 2.0 * sizeof(T) + 16.0 * f64));
+            hipLaunchKernelGGL((k_xfwd_rhs<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, I(), sI(), c->fs, ia, ib, Y,
+                               sY(), scale, ktab(c), rk, twx());
+        });
+        return HOS_OK;
+    }
+
+    // level 0: eta, eta_x, eta_y, phi_x, phi_y, d^j phi/dz^j (j = 1..M), the
+    // outputs of the first x inverse pass; returns their count
+    int level0(std::vector<XinvOp> &ops, FieldMap &fm)
+    {
+        const HosIdx &ix = c->ix;
+        int n = 0;
+        auto add = [&](int src, int op, int pw, int pdst) {
+            ops.push_back({src, op, pw, n});
+            fm.src[n] = n;
+            fm.dst[n] = pdst;
+            ++n;
+        };
+        add(0, OP_ID, 0, ix.eta);
+        add(0, OP_DX, 0, ix.etax);
+        add(0, OP_DY, 0, ix.etay);
+        add(1, OP_DX, 0, ix.phix);
+        add(1, OP_DY, 0, ix.phiy);
+        for (int j = 1; j <= c->M; ++j) add(1, OP_DZ, j, ix.d[1][j]);
+        return n;
+    }
+
+    // y r2c of phi^(lev+1) (prologue instantiated per level) and the RHS
+    // pair pass (prologue instantiated per order); defined in hos_levels.hip
+    int phi_level(int lev);
+    int rhs_pair();
+
+    // physical fields of the W expansion from the stage input Y
+    int w_levels(const cx<double> *Y)
+    {
+        const HosIdx &ix = c->ix;
+        const int M = c->M;
+        std::vector<XinvOp> ops;
+        FieldMap fm{};
+        const int n = level0(ops, fm);
+        int rc = xinv(Y, ops);
+        if (!rc) rc = yc2r(fm, n);
+        for (int lev = 1; lev <= M - 1 && !rc; ++lev) {
+            rc = phi_level(lev);
+            const int nd = M - lev;
+            if (!rc) rc = xfwdinv(0, 1, nd);
+            FieldMap f2{};
+            for (int j = 1; j <= nd; ++j) {
+                f2.src[j - 1] = j;
+                f2.dst[j - 1] = ix.d[lev + 1][j];
+            }
+            if (!rc) rc = yc2r(f2, nd);
+        }
+        return rc;
+    }
+
+    // RHS: the W levels (each field its own workgroups), the two RHS fields'
+    // y r2c, and the x forward pass with the RK epilogue.  (Fusing a level's
+    // y c2r with the next y r2c in one kernel serialises the fields of a
+    // level inside a workgroup and measured slower.)
+    int rhs(const cx<double> *Y, const RkArgs &rk)
+    {
+        int rc = w_levels(Y);
+        if (rc) return rc;
+        rc = rhs_pair();
+        if (!rc) rc = xfwd_rhs(0, 1, Y, rk);
+        return rc;
+    }
+};

@@ -28,14 +28,15 @@
 #include <vector>
 
 #include "../../include/hos_gpu.h"
-#include "hos_kernels.hpp"
+#include "hos_ctx.hpp"
+#include "hos_misc_kernels.hpp"
 
 using namespace hos;
 
 // ---------------------------------------------------------------------------
 static thread_local std::string g_err;
 
-static int fail(int code, const char *fmt, ...)
+int fail(int code, const char *fmt, ...)
 {
     char b[512];
     va_list ap;
@@ -46,12 +47,6 @@ static int fail(int code, const char *fmt, ...)
     return code;
 }
 
-#define HIP_TRY(expr)                                                                              \
-    do {                                                                                           \
-        hipError_t e_ = (expr);                                                                    \
-        if (e_ != hipSuccess) return fail(HOS_E_DEVICE, "%s: %s", #expr, hipGetErrorString(e_));   \
-    } while (0)
-
 static int ilog2(int n)
 {
     int l = 0;
@@ -59,91 +54,6 @@ static int ilog2(int n)
     return (1 << l) == n ? l : -1;
 }
 
-constexpr int LOGN_MIN = 4, LOGN_MAX = 11;
-
-struct hos_ctx {
-    hos_params p{};
-    int nm = 0, M = 0, L = 0, Nk = 0, lognx = 0, logny = 0, mx = 0, my = 0;
-    int nI = 0, nP = 0;
-    int tx[3] = {8, 4, 4};  // y tiles: c2r, r2c, pair
-    bool f32 = false;
-    long fs = 0;    // complex elements per spectral field  (Nk*Nx)
-    long fsP = 0;   // real elements per physical field     (Nx*Ny)
-    hipStream_t stream = nullptr;
-    // device memory
-    cx<double> *U = nullptr, *A = nullptr, *Y = nullptr, *K = nullptr;
-    void *I = nullptr, *P = nullptr;
-    void *twx = nullptr, *twy = nullptr;
-    double *kx = nullptr, *t = nullptr, *dt = nullptr, *H = nullptr, *vol = nullptr, *hpart = nullptr,
-           *mean0 = nullptr, *depth = nullptr;
-    int active = 0;  // members 0..active-1 are stepped
-    std::vector<hipGraphExec_t> gexec_by_active;
-    std::vector<hipGraph_t> graph_by_active;
-    bool async_pending = false;
-    int async_steps = 0;
-    double *hostbuf = nullptr;  // device staging [Nx][Ny] double
-    int nK = 0;                 // number of FP64 [2][Nk][Nx] slots in K (per member for RK)
-    HosIdx ix{};
-    // Butcher table, time_schemes_par.c:127-314
-    int S = 0;
-    double a[7] = {}, b[7][7] = {}, c[7] = {};
-    bool subdiag = true;
-    // timing / graph
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double last_ms = 0.0;
-    int use_graph = 0;
-    std::vector<double> host_t;   // host mirror of t
-    std::vector<double> host_dt;  // host mirror of dt
-    // per-step and per-kernel HIP-event timing
-    std::vector<hipEvent_t> step_ev;
-    std::vector<double> step_ms;
-    int prof_on = 0;
-    bool capturing = false;
-    struct ProfRec {
-        int kind;
-        hipEvent_t a, b;
-        double bytes;
-    };
-    std::vector<ProfRec> recs;
-    std::vector<hipEvent_t> evpool;
-    size_t evused = 0;
-    double kms[HOS_K_NKINDS] = {}, kbytes[HOS_K_NKINDS] = {};
-    long kn[HOS_K_NKINDS] = {};
-};
-
-static hipEvent_t pool_event(hos_ctx *c)
-{
-    if (c->evused < c->evpool.size()) return c->evpool[c->evused++];
-    hipEvent_t e = nullptr;
-    (void)hipEventCreate(&e);
-    c->evpool.push_back(e);
-    c->evused++;
-    return e;
-}
-
-// Records a HIP event pair around one kernel launch on the context's stream
-// when kernel profiling is on (eager launches only).
-struct Prof {
-    hos_ctx *c;
-    int kind;
-    double bytes;
-    hipEvent_t a = nullptr;
-    Prof(hos_ctx *c_, int k, double by) : c(c_), kind(k), bytes(by)
-    {
-        if (c->prof_on && !c->capturing) {
-            a = pool_event(c);
-            (void)hipEventRecord(a, c->stream);
-        }
-    }
-    ~Prof()
-    {
-        if (a) {
-            hipEvent_t b = pool_event(c);
-            (void)hipEventRecord(b, c->stream);
-            c->recs.push_back({kind, a, b, bytes});
-        }
-    }
-};
 
 static void harvest_prof(hos_ctx *c)
 {
@@ -289,6 +199,7 @@ static size_t footprint(const hos_ctx &c)
     b += (size_t)c.nm * c.nI * c.fs * 2 * rb;      // I
     b += (size_t)c.nm * c.nP * c.fsP * rb;         // P
     b += (size_t)c.fsP * 8;                        // staging
+    b += (size_t)c.nm * c.fs * 8;                  // tanh(|k| h) tables
     return b;
 }
 
@@ -321,12 +232,13 @@ static int upload_twiddles(int n, void **dst)
 }
 
 static void drop_graphs(hos_ctx *c);
+static int th_table(hos_ctx *c, int m);
 
 static void release(hos_ctx *c)
 {
     drop_graphs(c);
     void *ptrs[] = {c->U, c->A, c->Y, c->K, c->I, c->P, c->twx, c->twy, c->kx, c->t, c->dt, c->H, c->vol,
-                    c->hpart, c->mean0, c->depth, c->hostbuf};
+                    c->hpart, c->mean0, c->depth, c->th, c->hostbuf};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
@@ -385,6 +297,7 @@ extern "C" int hos_create(const hos_params *p, int nmembers, hos_ctx **out)
                   {(void **)&c->vol, sizeof(double) * nm},
                   {(void **)&c->mean0, sizeof(double) * nm},
                   {(void **)&c->depth, sizeof(double) * nm},
+                  {(void **)&c->th, sizeof(double) * nm * c->fs},
                   {(void **)&c->hpart, sizeof(double) * nm * nparts},
                   {(void **)&c->hostbuf, sizeof(double) * c->fsP}};
     for (auto &al : allocs) {
@@ -392,6 +305,8 @@ extern "C" int hos_create(const hos_params *p, int nmembers, hos_ctx **out)
             return bail(fail(HOS_E_NOMEM, "hipMalloc of %zu bytes failed", al.n));
         (void)hipMemset(*al.p, 0, al.n);
     }
+    // the clears above run on the null stream; the context stream does not wait for it
+    if (hipDeviceSynchronize() != hipSuccess) return bail(fail(HOS_E_DEVICE, "device sync failed"));
     // kx table, operators_par.c:55-63 (reference) or signed
     std::vector<double> kx(p->Nx);
     const double Kx0 = 2 * M_PI / p->Lx;
@@ -413,6 +328,8 @@ extern "C" int hos_create(const hos_params *p, int nmembers, hos_ctx **out)
         std::vector<double> dep(nm, p->depth);
         if (hipMemcpy(c->depth, dep.data(), sizeof(double) * nm, hipMemcpyHostToDevice) != hipSuccess)
             return bail(fail(HOS_E_DEVICE, "depth upload failed"));
+        for (int m = 0; m < nm; ++m)
+            if ((rc = th_table(c, m))) return bail(rc);
     }
     c->gexec_by_active.assign(nm + 1, nullptr);
     c->graph_by_active.assign(nm + 1, nullptr);
@@ -433,273 +350,6 @@ extern "C" int hos_destroy(hos_ctx *c)
 extern "C" int hos_num_members(hos_ctx *c) { return c ? c->nm : 0; }
 
 // ---------------------------------------------------------------------------
-// kernel dispatch on (T, log2 N, tile)
-static KTab ktab(const hos_ctx *c)
-{
-    KTab k;
-    k.kx = c->kx;
-    k.ky0 = 2 * M_PI / c->p.Ly;
-    k.depth = c->depth;
-    k.nk = c->Nk;
-    k.mx = c->mx;
-    k.my = c->my;
-    return k;
-}
-
-#define LOGN_SWITCH(v, ...)                                                                       \
-    switch (v) {                                                                                   \
-    case 4: { constexpr int LG = 4; __VA_ARGS__; } break;                                                 \
-    case 5: { constexpr int LG = 5; __VA_ARGS__; } break;                                                 \
-    case 6: { constexpr int LG = 6; __VA_ARGS__; } break;                                                 \
-    case 7: { constexpr int LG = 7; __VA_ARGS__; } break;                                                 \
-    case 8: { constexpr int LG = 8; __VA_ARGS__; } break;                                                 \
-    case 9: { constexpr int LG = 9; __VA_ARGS__; } break;                                                 \
-    case 10: { constexpr int LG = 10; __VA_ARGS__; } break;                                               \
-    case 11: { constexpr int LG = 11; __VA_ARGS__; } break;                                               \
-    default: return fail(HOS_E_INVALID, "unsupported size 2^%d", v);                                \
-    }
-
-// TXV = y tile; only tiles whose LDS fits 64 KiB are instantiated
-#define TX_CASE(T, V, ...)                                                                         \
-    case V: {                                                                                      \
-        constexpr int TXV = V;                                                                     \
-        if constexpr ((TXV / 2) * (1 << LG) * sizeof(cx<T>) <= 65536) {                            \
-            __VA_ARGS__;                                                                           \
-        } else {                                                                                   \
-            return fail(HOS_E_INVALID, "tile %d exceeds 64 KiB of LDS at Ny = %d", V, 1 << LG);     \
-        }                                                                                          \
-    } break;
-#define TX_SWITCH(T, tx, ...)                                                                      \
-    switch (tx) {                                                                                  \
-        TX_CASE(T, 4, __VA_ARGS__)                                                                 \
-        TX_CASE(T, 8, __VA_ARGS__)                                                                 \
-        TX_CASE(T, 16, __VA_ARGS__)                                                                \
-    default: return fail(HOS_E_INVALID, "unsupported tile %d", tx);                                \
-    }
-
-template <class T>
-struct Eng {
-    hos_ctx *c;
-    int nm;  // members in this launch (from member 0)
-    cx<T> *I() const { return (cx<T> *)c->I; }
-    T *P() const { return (T *)c->P; }
-    long sI() const { return (long)c->nI * c->fs; }
-    long sP() const { return (long)c->nP * c->fsP; }
-    long sY() const { return 2 * c->fs; }
-    const cx<T> *twx() const { return (const cx<T> *)c->twx; }
-    const cx<T> *twy() const { return (const cx<T> *)c->twy; }
-    Phys<T> phys() const { return Phys<T>{P(), sP(), c->fsP, c->p.Ny}; }
-    bool comp() const { return c->f32 && c->p.compensated; }
-
-    // outputs list[i] (dst = intermediate index), grouped by source into
-    // chunks of at most `per` outputs, one chunk per blockIdx.y.  Few rows in
-    // flight (one member): small chunks for more workgroups; many members:
-    // one chunk per source, so each source row is read once.
-    int xinv(const cx<double> *Y, const std::vector<XinvOp> &list, int per = 0)
-    {
-        if (per <= 0) per = ((long)nm * c->Nk >= 4096) ? MAXPERGRP : 3;
-        XinvOps ops{};
-        for (int s = 0; s < 2; ++s) {
-            int k = 0;
-            for (const XinvOp &o : list) {
-                if (o.src != s) continue;
-                if (k == 0) {
-                    if (ops.ngroups == MAXGRP) return fail(HOS_E_INVALID, "too many x-inverse outputs");
-                    ops.src[ops.ngroups] = s;
-                    ops.n[ops.ngroups] = 0;
-                    ops.ngroups++;
-                }
-                const int g = ops.ngroups - 1;
-                ops.o[g][ops.n[g]++] = o;
-                k = (k + 1) % std::min(per, MAXPERGRP);
-            }
-        }
-        const int ngroups = ops.ngroups;
-        const int nops = (int)list.size();
-        LOGN_SWITCH(c->lognx, {
-            constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
-            dim3 grid((c->Nk + ROWS - 1) / ROWS, ngroups, nm);
-            Prof pf(c, HOS_K_XINV, (double)nm * c->fs * (16.0 * ngroups + 2.0 * sizeof(T) * nops));
-            hipLaunchKernelGGL((k_xinv<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, Y, sY(), I(), sI(), c->fs, ops,
-                               ktab(c), twx());
-        });
-        return HOS_OK;
-    }
-    int yc2r(const FieldMap &fm, int nf)
-    {
-        const int tx = c->tx[HOS_TILE_C2R];
-        LOGN_SWITCH(c->logny, {
-            TX_SWITCH(T, tx, {
-                dim3 grid(c->p.Nx / TXV, nf, nm);
-                Prof pf(c, HOS_K_YC2R, (double)nm * nf * sizeof(T) * (2.0 * c->fs + c->fsP));
-                hipLaunchKernelGGL((k_yc2r<T, LG, TXV>), grid, dim3(NT), 0, c->stream, I(), sI(), c->fs, P(), sP(),
-                                   c->fsP, fm, c->p.Nx, twy());
-            })
-        });
-        return HOS_OK;
-    }
-    template <class PRO>
-    int yr2c(const PRO &pro, int dst, int nin)
-    {
-        const int tx = c->tx[HOS_TILE_R2C];
-        LOGN_SWITCH(c->logny, {
-            TX_SWITCH(T, tx, {
-                dim3 grid(c->p.Nx / TXV, 1, nm);
-                Prof pf(c, HOS_K_YR2C, (double)nm * sizeof(T) * (nin * (double)c->fsP + 2.0 * c->fs));
-                hipLaunchKernelGGL((k_yr2c<T, LG, TXV, PRO>), grid, dim3(NT), 0, c->stream, pro, I(), sI(), c->fs,
-                                   dst, c->p.Nx, twy());
-            })
-        });
-        return HOS_OK;
-    }
-    template <class PRO>
-    int yr2c_pair(const PRO &pro, int da, int db, int nin)
-    {
-        const int tx = c->tx[HOS_TILE_PAIR];
-        LOGN_SWITCH(c->logny, {
-            TX_SWITCH(T, tx, {
-                constexpr int TXP = TXV / 2;
-                dim3 grid(c->p.Nx / TXP, 1, nm);
-                Prof pf(c, HOS_K_YR2C_PAIR, (double)nm * sizeof(T) * (nin * (double)c->fsP + 4.0 * c->fs));
-                hipLaunchKernelGGL((k_yr2c_pair<T, LG, TXP, PRO>), grid, dim3(NT), 0, c->stream, pro, I(), sI(),
-                                   c->fs, da, db, c->p.Nx, twy());
-            })
-        });
-        return HOS_OK;
-    }
-    int xfwdinv(int src, int dst0, int ndst)
-    {
-        const double scale = 1.0 / ((double)c->p.Nx * c->p.Ny);
-        LOGN_SWITCH(c->lognx, {
-            constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
-            dim3 grid((c->Nk + ROWS - 1) / ROWS, 1, nm);
-            Prof pf(c, HOS_K_XFWDINV, (double)nm * c->fs * 2.0 * sizeof(T) * (1 + ndst));
-            hipLaunchKernelGGL((k_xfwdinv<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, I(), sI(), c->fs, src, dst0,
-                               ndst, scale, ktab(c), twx());
-        });
-        return HOS_OK;
-    }
-    int xfwd_rhs(int ia, int ib, const cx<double> *Y, const RkArgs &rk)
-    {
-        const double scale = 1.0 / ((double)c->p.Nx * c->p.Ny);
-        // FP64 [2][Nk][Nx] fields moved by the epilogue
-        double f64 = 0.0;
-        if (rk.mode == EPI_RK_SUBDIAG)
-            f64 = 2 /*Y*/ + 2 /*u*/ + (rk.stage > 0 ? 2 : 0) /*acc*/ + 2 /*acc or u*/ + 2 /*Y next*/;
-        else if (rk.mode == EPI_STORE_K)
-            f64 = 2 + 2 + (rk.hpart && rk.stage == 0 ? 2 : 0);
-        else
-            f64 = 2;
-        LOGN_SWITCH(c->lognx, {
-            constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
-            dim3 grid((c->Nk + ROWS - 1) / ROWS, 1, nm);
-            Prof pf(c, HOS_K_XFWD_RHS, (double)nm * c->fs * (2.0 * 2.0 * sizeof(T) + 16.0 * f64));
-            hipLaunchKernelGGL((k_xfwd_rhs<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, I(), sI(), c->fs, ia, ib, Y,
-                               sY(), scale, ktab(c), rk, twx());
-        });
-        return HOS_OK;
-    }
-
-    // level 0: eta, eta_x, eta_y, phi_x, phi_y, d^j phi/dz^j (j = 1..M), the
-    // outputs of the first x inverse pass; returns their count
-    int level0(std::vector<XinvOp> &ops, FieldMap &fm)
-    {
-        const HosIdx &ix = c->ix;
-        int n = 0;
-        auto add = [&](int src, int op, int pw, int pdst) {
-            ops.push_back({src, op, pw, n});
-            fm.src[n] = n;
-            fm.dst[n] = pdst;
-            ++n;
-        };
-        add(0, OP_ID, 0, ix.eta);
-        add(0, OP_DX, 0, ix.etax);
-        add(0, OP_DY, 0, ix.etay);
-        add(1, OP_DX, 0, ix.phix);
-        add(1, OP_DY, 0, ix.phiy);
-        for (int j = 1; j <= c->M; ++j) add(1, OP_DZ, j, ix.d[1][j]);
-        return n;
-    }
-
-    // y r2c of phi^(lev+1), the prologue instantiated for each level
-    template <int LEV>
-    int phi_level_t()
-    {
-        ProPhi<T, LEV> pro{phys(), c->ix, comp()};
-        return yr2c(pro, 0, LEV + 1);
-    }
-    int phi_level(int lev)
-    {
-        switch (lev) {
-        case 1: return phi_level_t<1>();
-        case 2: return phi_level_t<2>();
-        case 3: return phi_level_t<3>();
-        case 4: return phi_level_t<4>();
-        case 5: return phi_level_t<5>();
-        case 6: return phi_level_t<6>();
-        case 7: return phi_level_t<7>();
-        }
-        static_assert(MAXM == 8, "levels 1..MAXM-1");
-        return fail(HOS_E_INVALID, "level %d out of range", lev);
-    }
-    template <int MO>
-    int rhs_pair_t()
-    {
-        ProRhs<T, MO> pro{phys(), c->ix, comp()};
-        return yr2c_pair(pro, 0, 1, c->nP);
-    }
-    int rhs_pair()
-    {
-        switch (c->M) {
-        case 1: return rhs_pair_t<1>();
-        case 2: return rhs_pair_t<2>();
-        case 3: return rhs_pair_t<3>();
-        case 4: return rhs_pair_t<4>();
-        case 5: return rhs_pair_t<5>();
-        case 6: return rhs_pair_t<6>();
-        case 7: return rhs_pair_t<7>();
-        case 8: return rhs_pair_t<8>();
-        }
-        return fail(HOS_E_INVALID, "order %d out of range", c->M);
-    }
-
-    // physical fields of the W expansion from the stage input Y
-    int w_levels(const cx<double> *Y)
-    {
-        const HosIdx &ix = c->ix;
-        const int M = c->M;
-        std::vector<XinvOp> ops;
-        FieldMap fm{};
-        const int n = level0(ops, fm);
-        int rc = xinv(Y, ops);
-        if (!rc) rc = yc2r(fm, n);
-        for (int lev = 1; lev <= M - 1 && !rc; ++lev) {
-            rc = phi_level(lev);
-            const int nd = M - lev;
-            if (!rc) rc = xfwdinv(0, 1, nd);
-            FieldMap f2{};
-            for (int j = 1; j <= nd; ++j) {
-                f2.src[j - 1] = j;
-                f2.dst[j - 1] = ix.d[lev + 1][j];
-            }
-            if (!rc) rc = yc2r(f2, nd);
-        }
-        return rc;
-    }
-
-    // RHS: the W levels (each field its own workgroups), the two RHS fields'
-    // y r2c, and the x forward pass with the RK epilogue.  (Fusing a level's
-    // y c2r with the next y r2c in one kernel serialises the fields of a
-    // level inside a workgroup and measured slower.)
-    int rhs(const cx<double> *Y, const RkArgs &rk)
-    {
-        int rc = w_levels(Y);
-        if (rc) return rc;
-        rc = rhs_pair();
-        if (!rc) rc = xfwd_rhs(0, 1, Y, rk);
-        return rc;
-    }
-};
 
 static RkArgs rk_base(hos_ctx *c)
 {
@@ -899,6 +549,17 @@ extern "C" int hos_set_active(hos_ctx *c, int n)
 
 extern "C" int hos_get_active(hos_ctx *c) { return c ? c->active : 0; }
 
+// tanh(|k| h) of member m from its depth (device, FP64), synchronous
+static int th_table(hos_ctx *c, int m)
+{
+    const long n = c->fs;
+    hipLaunchKernelGGL(k_th_table, dim3((unsigned)((n + NT - 1) / NT)), dim3(NT), 0, c->stream, ktab(c), c->p.Nx, m,
+                       c->th);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return HOS_OK;
+}
+
 extern "C" int hos_set_member(hos_ctx *c, int m, double depth, double dt, double t)
 {
     if (!c) return fail(HOS_E_INVALID, "null context");
@@ -909,7 +570,7 @@ extern "C" int hos_set_member(hos_ctx *c, int m, double depth, double dt, double
     HIP_TRY(hipMemcpy(c->depth + m, &depth, sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->dt + m, &c->host_dt[m], sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->t + m, &c->host_t[m], sizeof(double), hipMemcpyHostToDevice));
-    return HOS_OK;
+    return th_table(c, m);
 }
 
 extern "C" int hos_copy_member(hos_ctx *c, int src, int dst)
@@ -924,6 +585,8 @@ extern "C" int hos_copy_member(hos_ctx *c, int src, int dst)
     double *scal[] = {c->t, c->dt, c->depth, c->mean0, c->H, c->vol};
     for (double *a : scal)
         HIP_TRY(hipMemcpyAsync(a + dst, a + src, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->th + dst * c->fs, c->th + src * c->fs, c->fs * sizeof(double), hipMemcpyDeviceToDevice,
+                           c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->host_t[dst] = c->host_t[src];
     c->host_dt[dst] = c->host_dt[src];

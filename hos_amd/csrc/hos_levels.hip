@@ -1,0 +1,60 @@
+// hos_levels.hip -- the y r2c launches whose prologue is instantiated per
+// HOS level (phi^(lev+1), lev = 1..7) or per order (the RHS pair, M = 1..8).
+// Compiled once per precision (HOS_T) and part (HOS_PART 0: levels, 1: RHS)
+// by the Makefile, so the several hundred kernel instantiations build in
+// parallel translation units.
+#include "hos_ctx.hpp"
+
+#ifndef HOS_T
+#error "the Makefile sets HOS_T (float or double) and HOS_PART (0 or 1)"
+#endif
+
+#if HOS_PART == 0
+template <class T, int LEV>
+static int phi_level_t(Eng<T> &e)
+{
+    ProPhi<T, LEV> pro{e.phys(), e.c->ix, e.comp()};
+    return e.yr2c(pro, 0, LEV + 1);
+}
+
+template <class T>
+int Eng<T>::phi_level(int lev)
+{
+    static_assert(MAXM == 8, "levels 1..MAXM-1");
+    switch (lev) {
+    case 1: return phi_level_t<T, 1>(*this);
+    case 2: return phi_level_t<T, 2>(*this);
+    case 3: return phi_level_t<T, 3>(*this);
+    case 4: return phi_level_t<T, 4>(*this);
+    case 5: return phi_level_t<T, 5>(*this);
+    case 6: return phi_level_t<T, 6>(*this);
+    case 7: return phi_level_t<T, 7>(*this);
+    }
+    return fail(HOS_E_INVALID, "level %d out of range", lev);
+}
+template int Eng<HOS_T>::phi_level(int);
+#else
+template <class T, int MO>
+static int rhs_pair_t(Eng<T> &e)
+{
+    ProRhs<T, MO> pro{e.phys(), e.c->ix, e.comp()};
+    return e.yr2c_pair(pro, 0, 1, e.c->nP);
+}
+
+template <class T>
+int Eng<T>::rhs_pair()
+{
+    switch (c->M) {
+    case 1: return rhs_pair_t<T, 1>(*this);
+    case 2: return rhs_pair_t<T, 2>(*this);
+    case 3: return rhs_pair_t<T, 3>(*this);
+    case 4: return rhs_pair_t<T, 4>(*this);
+    case 5: return rhs_pair_t<T, 5>(*this);
+    case 6: return rhs_pair_t<T, 6>(*this);
+    case 7: return rhs_pair_t<T, 7>(*this);
+    case 8: return rhs_pair_t<T, 8>(*this);
+    }
+    return fail(HOS_E_INVALID, "order %d out of range", c->M);
+}
+template int Eng<HOS_T>::rhs_pair();
+#endif
