@@ -138,13 +138,17 @@ inline KTab ktab(const hos_ctx *c)
     }
 
 // TXV = y tile; only tiles whose LDS fits 64 KiB are instantiated
+// LDS a y tile may use: 64 KiB, or all 160 KiB of a gfx950 CU for the
+// narrowest tile (FP64 rows of 2048)
+constexpr int tile_lds_max(int tx) { return tx == 4 ? 163840 : 65536; }
+
 #define TX_CASE(T, V, ...)                                                                         \
     case V: {                                                                                      \
         constexpr int TXV = V;                                                                     \
-        if constexpr ((TXV / 2) * (1 << LG) * sizeof(cx<T>) <= 65536) {                            \
+        if constexpr (Lds<T, (1 << LG)>::size(TXV / 2) * sizeof(cx<T>) <= tile_lds_max(TXV)) {     \
             __VA_ARGS__;                                                                           \
         } else {                                                                                   \
-            return fail(HOS_E_INVALID, "tile %d exceeds 64 KiB of LDS at Ny = %d", V, 1 << LG);     \
+            return fail(HOS_E_INVALID, "tile %d exceeds the LDS limit at Ny = %d", V, 1 << LG);     \
         }                                                                                          \
     } break;
 #define TX_SWITCH(T, tx, ...)                                                                      \

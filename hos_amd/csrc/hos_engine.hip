@@ -128,12 +128,13 @@ static int setup_scheme(hos_ctx *c, int flg)
 }
 
 // y-pass tiles: TX x-rows per workgroup, TX/2 complex rows of Ny in LDS
-// (<= 64 KiB).  Wider tiles write longer transposed segments; narrower ones
+// (tile_lds_max).  Wider tiles write longer transposed segments; narrower ones
 // give more workgroups when a launch carries few fields.
 static bool tx_fits(bool f32, int logny, int tx)
 {
-    const int bytes = f32 ? 8 : 16;
-    return (tx == 4 || tx == 8 || tx == 16) && (tx / 2) * (1 << logny) * bytes <= 65536;
+    // LDS rows in the Lds<T, N> layout: N + N/16 elements plus 128 bytes each
+    const int bytes = f32 ? 8 : 16, n = 1 << logny;
+    return (tx == 4 || tx == 8 || tx == 16) && (tx / 2) * ((n + n / 16) * bytes + 128) <= tile_lds_max(tx);
 }
 
 static int default_tx(bool f32, int logny, int kind)

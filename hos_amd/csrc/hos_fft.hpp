@@ -5,8 +5,9 @@
 // share the workgroup, 8*8*4*4 for one row), all threads of the workgroup
 // cooperating on NROWS rows at once.  Each stage reads its R inputs into
 // registers, does the R-point DFT there (radix 16 = 4 x 4), barriers, and
-// writes R outputs in place, so one buffer of NROWS*N complex values is all
-// the LDS a transform needs and each stage is one LDS round trip.
+// writes R outputs in place, so one buffer of NROWS rows (Lds<T, N> layout,
+// padded against bank conflicts) is all the LDS a transform needs and each
+// stage is one LDS round trip.
 // Twiddles come from a per-length table tw[k] = exp(-2 pi i k/N) computed on
 // the host in long double; the inverse uses the conjugate.  Transforms are
 // unnormalised (the 1/(Nx*Ny) of fft_2d is applied by the r2c x-pass).
@@ -29,6 +30,18 @@ template <class T>
 __device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) { return {a.x + b.x, a.y + b.y}; }
 template <class T>
 __device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { return {a.x - b.x, a.y - b.y}; }
+
+// LDS layout of rows of N complex values: one pad element after every 16, so
+// the strided Stockham writes (stride R = 16 in the first stage) land in
+// distinct banks, and a row pitch of N + N/16 plus 128 bytes.
+template <class T, int N>
+struct Lds {
+    static_assert(N >= 16, "rows of at least 16");
+    static constexpr int SH = 128 / (int)sizeof(cx<T>);
+    static constexpr int PITCH = N + N / 16 + SH;
+    static constexpr int size(int rows) { return rows * PITCH; }
+    __device__ static __forceinline__ int at(int row, int i) { return row * PITCH + i + (i >> 4); }
+};
 
 // Stage plan for NROWS rows of length 2^logn: radix 16, 8, 4 or 2 per stage,
 // the largest radix that keeps at least min(128, NROWS*N/4) butterflies per
@@ -149,15 +162,19 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, const cx<T> *__restrict__ 
     constexpr int NR = N / R;
     constexpr int NBF = NROWS * NR;
     constexpr int Q = (NBF + NT - 1) / NT;
+    using L = Lds<T, N>;
     cx<T> v[Q][R];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int bf = threadIdx.x + q * NT;
         if ((NBF % NT) == 0 || bf < NBF) {
             const int row = bf / NR, j = bf % NR;
-            const cx<T> *src = buf + row * N;
+            // L::at(row, j + r NR) as a per-thread base plus a constant per r:
+            // NR is a power of two and j < NR, so (j + r NR) >> 4 equals
+            // (j >> 4) + ((r NR) >> 4)
+            const int rb = row * L::PITCH + j + (j >> 4);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = src[j + r * NR];
+            for (int r = 0; r < R; ++r) v[q][r] = buf[rb + r * NR + ((r * NR) >> 4)];
             if constexpr (NS > 1) {
                 // w^r, r = 1..R-1, w = exp(-+2 pi i k/(NS R)): powers below 4 by
                 // repeated products of w, higher ones as (w^4)^q * w^b with w^4 a
@@ -201,9 +218,11 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, const cx<T> *__restrict__ 
         if ((NBF % NT) == 0 || bf < NBF) {
             const int row = bf / NR, j = bf % NR;
             const int idxD = (j / NS) * NS * R + (j % NS);
-            cx<T> *dst = buf + row * N;
+            // likewise (idxD + r NS) >> 4 = (idxD >> 4) + ((r NS) >> 4): idxD is a
+            // multiple of NS R plus j % NS < NS
+            const int wb = row * L::PITCH + idxD + (idxD >> 4);
 #pragma unroll
-            for (int r = 0; r < R; ++r) dst[idxD + r * NS] = v[q][r];
+            for (int r = 0; r < R; ++r) buf[wb + r * NS + ((r * NS) >> 4)] = v[q][r];
         }
     }
     __syncthreads();
@@ -219,9 +238,9 @@ __device__ __forceinline__ void fft_stages(cx<T> *buf, const cx<T> *__restrict__
     }
 }
 
-// In-place FFT of NROWS contiguous rows of length 2^LOGN held in LDS.  Starts
-// and ends with a workgroup barrier, so callers may write buf just before and
-// read it just after.
+// In-place FFT of NROWS rows of length 2^LOGN held in LDS in the Lds<T, N>
+// layout.  Starts and ends with a workgroup barrier, so callers may write buf
+// just before and read it just after.
 template <class T, int LOGN, int NROWS, int NT, bool INV>
 __device__ __forceinline__ void lds_fft(cx<T> *buf, const cx<T> *__restrict__ tw)
 {

@@ -72,10 +72,27 @@ struct KTab {
     const double *kx;     // [Nx]
     double ky0;
     const double *depth;  // [members]; <= 0: deep water
+    const double *th;     // [members][Nk][Nx] tanh(|k| h), 1 in deep water (k_th_table)
+    long ths;             // member stride of th
     int nk;               // Ny/2+1
     int mx, my;           // dealias bounds, operators_par.c:290-291
     __device__ __forceinline__ double h(int m) const { return depth[m]; }
 };
+
+// |k| and tanh(|k| h) of one mode in the precision R of the multipliers: FP64
+// from the table (thd, loaded by the caller with its other loads); FP32 computed
+// on the fly, since the FP32 form rounds every multiplied value to FP32.
+template <class R>
+__device__ __forceinline__ void kz_th(double kxv, double kyv, double h, double thd, R &kz, R &th)
+{
+    if constexpr (sizeof(R) == 8) {
+        kz = sqrt(kxv * kxv + kyv * kyv);
+        th = thd;
+    } else {
+        kz = sqrtf(float(kxv * kxv + kyv * kyv));
+        th = h > 0.0 ? tanhf(kz * float(h)) : 1.0f;
+    }
+}
 
 // d^p/dz^p multiplier at |k| = kz: kz^p (deep), kz^p tanh(kz h) for odd p.
 __device__ __forceinline__ double dz_mul(double kz, int p, double depth)
@@ -144,29 +161,35 @@ __global__ __launch_bounds__(NT) void k_xinv(const cx<double> *__restrict__ Y, l
 {
     using G = XGeom<T, LOGNX>;
     static_assert(G::ROWS == ROWS, "x rows");
-    constexpr int NX = G::NX, NE = G::NE, PER = G::PER, NB = G::XNB;
-    __shared__ cx<T> buf[NB * NE];
-    const int grp = blockIdx.y;
+    constexpr int NX = G::NX, PER = G::PER, NB = G::XNB;
+    using R = T;  // multiplier arithmetic: FP64 (FP64 form) or FP32 (FP32 form)
+    using L = Lds<T, NX>;
+    __shared__ cx<T> buf[L::size(NB * ROWS)];
+    const int grp = blockIdx.y, m = blockIdx.z;
     const int ky0 = blockIdx.x * ROWS;
-    const cx<double> *src = Y + blockIdx.z * sY + ops.src[grp] * fs;
+    const cx<double> *src = Y + m * sY + ops.src[grp] * fs;
+    const double *tht = kt.th + m * kt.ths;
+    const double h = kt.h(m);
     cx<double> s[PER];
-    double kxv[PER], kz[PER], th[PER];
-    const double h = kt.h(blockIdx.z);
+    double kxv[PER], thd[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int e = threadIdx.x + q * NT;
         const int kx = e % NX, ky = min(ky0 + e / NX, kt.nk - 1);
         s[q] = src[(long)ky * NX + kx];
         kxv[q] = kt.kx[kx];
+        if constexpr (sizeof(R) == 8) thd[q] = tht[(long)ky * NX + kx];
+        else thd[q] = 1.0;
     }
+    cx<R> sr[PER];
+    R kxr[PER], kz[PER], th[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int e = threadIdx.x + q * NT;
         const int ky = ky0 + e / NX;
-        if (ky >= kt.nk) s[q] = {0.0, 0.0};
-        const double kyv = ky * kt.ky0;
-        kz[q] = sqrt(kxv[q] * kxv[q] + kyv * kyv);
-        th[q] = h > 0.0 ? tanh(kz[q] * h) : 1.0;  // d^n/dz^n at finite depth: k^n tanh(kh)^(n odd)
+        sr[q] = ky < kt.nk ? cx<R>{R(s[q].x), R(s[q].y)} : cx<R>{R(0), R(0)};
+        kxr[q] = R(kxv[q]);
+        kz_th<R>(kxv[q], ky * kt.ky0, h, thd[q], kz[q], th[q]);  // d^n/dz^n: k^n tanh(kh)^(n odd)
     }
     cx<T> *base = I + blockIdx.z * sI;
     const int nops = ops.n[grp];
@@ -180,22 +203,22 @@ __global__ __launch_bounds__(NT) void k_xinv(const cx<double> *__restrict__ Y, l
             for (int q = 0; q < PER; ++q) {
                 const int e = threadIdx.x + q * NT;
                 const int ky = ky0 + e / NX;
-                double re = s[q].x, im = s[q].y;
+                R re = sr[q].x, im = sr[q].y;
                 if (op.op == OP_DX) {
-                    re = -kxv[q] * s[q].y;
-                    im = kxv[q] * s[q].x;
+                    re = -kxr[q] * sr[q].y;
+                    im = kxr[q] * sr[q].x;
                 } else if (op.op == OP_DY) {
-                    const double k = ky * kt.ky0;
-                    re = -k * s[q].y;
-                    im = k * s[q].x;
+                    const R k = R(ky * kt.ky0);
+                    re = -k * sr[q].y;
+                    im = k * sr[q].x;
                 } else if (op.op == OP_DZ) {
-                    double m = 1.0;
-                    for (int i = 0; i < op.pow; ++i) m *= kz[q];
-                    if (op.pow & 1) m *= th[q];
-                    re = m * s[q].x;
-                    im = m * s[q].y;
+                    R mm = R(1);
+                    for (int i = 0; i < op.pow; ++i) mm *= kz[q];
+                    if (op.pow & 1) mm *= th[q];
+                    re = mm * sr[q].x;
+                    im = mm * sr[q].y;
                 }
-                buf[b * NE + e] = {T(re), T(im)};
+                buf[L::at(b * ROWS + e / NX, e % NX)] = {T(re), T(im)};
             }
         }
         lds_fft<T, LOGNX, ROWS * NB, NT, true>(buf, twx);
@@ -207,7 +230,7 @@ __global__ __launch_bounds__(NT) void k_xinv(const cx<double> *__restrict__ Y, l
             for (int q = 0; q < PER; ++q) {
                 const int e = threadIdx.x + q * NT;
                 const int kx = e % NX, ky = ky0 + e / NX;
-                if (ky < kt.nk) dst[(long)ky * NX + kx] = buf[b * NE + e];
+                if (ky < kt.nk) dst[(long)ky * NX + kx] = buf[L::at(b * ROWS + e / NX, kx)];
             }
         }
     }
@@ -231,27 +254,33 @@ template <class T, int LOGNY, int NPR>
 __device__ __forceinline__ void c2r_tile(const cx<T> *__restrict__ src, T *__restrict__ dst, int x0, int nx,
                                          cx<T> *buf, const cx<T> *__restrict__ twy)
 {
-    constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1, NL = NK * NPR, PERL = (NL + NT - 1) / NT;
+    constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1;
+    using L = Lds<T, NY>;
+    // task (p, ky): 16 consecutive ky of one row pair per 16 lanes, so the LDS
+    // side is contiguous while each instruction still reads whole 16-byte
+    // pieces of the same ky lines
+    constexpr int NL = ((NK + 15) / 16) * 16 * NPR, PERL = (NL + NT - 1) / NT;
     cx<T> A[PERL], B[PERL];
 #pragma unroll
     for (int q = 0; q < PERL; ++q) {
-        // past the end: reload the last element (in bounds, value unused)
-        const int e = min((int)threadIdx.x + q * NT, NL - 1);
-        const int p = e % NPR, ky = e / NPR;
+        const int e = threadIdx.x + q * NT;
+        const int p = (e / 16) % NPR;
+        // past the end: reload the last row (in bounds, value unused)
+        const int ky = min((e / (16 * NPR)) * 16 + e % 16, NK - 1);
         load2(src + (long)ky * nx + x0 + 2 * p, A[q], B[q]);
     }
 #pragma unroll
     for (int q = 0; q < PERL; ++q) {
         const int e = threadIdx.x + q * NT;
-        if ((NL % NT) == 0 || e < NL) {
-            const int p = e % NPR, ky = e / NPR;
+        const int p = (e / 16) % NPR, ky = (e / (16 * NPR)) * 16 + e % 16;
+        if (((NL % NT) == 0 || e < NL) && ky < NK) {
             cx<T> a = A[q], b = B[q];
             if (ky == 0 || ky == NY / 2) {
                 a.y = T(0);
                 b.y = T(0);
             }
-            buf[p * NY + ky] = {a.x - b.y, a.y + b.x};
-            if (ky > 0 && ky < NY / 2) buf[p * NY + NY - ky] = {a.x + b.y, b.x - a.y};
+            buf[L::at(p, ky)] = {a.x - b.y, a.y + b.x};
+            if (ky > 0 && ky < NY / 2) buf[L::at(p, NY - ky)] = {a.x + b.y, b.x - a.y};
         }
     }
     lds_fft<T, LOGNY, NPR, NT, true>(buf, twy);
@@ -266,7 +295,7 @@ __device__ __forceinline__ void c2r_tile(const cx<T> *__restrict__ src, T *__res
             F ra, rb;
 #pragma unroll
             for (int v = 0; v < V; ++v) {
-                const cx<T> z = buf[p * NY + y + v];
+                const cx<T> z = buf[L::at(p, y) + v];
                 ra[v] = z.x;
                 rb[v] = z.y;
             }
@@ -282,7 +311,7 @@ __global__ __launch_bounds__(NT) void k_yc2r(const cx<T> *__restrict__ I, long s
                                              const cx<T> *__restrict__ twy)
 {
     constexpr int NY = 1 << LOGNY, NPR = TX / 2;
-    __shared__ cx<T> buf[NPR * NY];
+    __shared__ cx<T> buf[Lds<T, NY>::size(NPR)];
     const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TX;
     c2r_tile<T, LOGNY, NPR>(I + blockIdx.z * sI + fm.src[blockIdx.y] * fsI,
                             P + blockIdx.z * sP + fm.dst[blockIdx.y] * fsP, x0, nx, buf, twy);
@@ -581,17 +610,20 @@ __device__ __forceinline__ void r2c_tile(const PRO &pro, int m, cx<T> *__restric
             const int p = e / NYV, y = (e % NYV) * V;
             const F ra = pro(m, x0 + 2 * p, y);
             const F rb = pro(m, x0 + 2 * p + 1, y);
-            cx<T> *d = buf + p * NY + y;
+            cx<T> *d = buf + Lds<T, NY>::at(p, y);
 #pragma unroll
             for (int v = 0; v < V; ++v) d[v] = {ra[v], rb[v]};
         }
     }
     lds_fft<T, LOGNY, NPR, NT, false>(buf, twy);
-    // the two real rows of pair p at ky from Z(ky) and Z(NY-ky)
-    for (int e = threadIdx.x; e < NK * NPR; e += NT) {
-        const int p = e % NPR, ky = e / NPR;
-        const cx<T> zk = buf[p * NY + ky];
-        const cx<T> zm = buf[p * NY + ((NY - ky) & (NY - 1))];
+    // the two real rows of pair p at ky from Z(ky) and Z(NY-ky); 16 consecutive
+    // ky of one pair per 16 lanes (contiguous in LDS)
+    constexpr int NO = ((NK + 15) / 16) * 16 * NPR;
+    for (int e = threadIdx.x; e < NO; e += NT) {
+        const int p = (e / 16) % NPR, ky = (e / (16 * NPR)) * 16 + e % 16;
+        if (ky >= NK) continue;
+        const cx<T> zk = buf[Lds<T, NY>::at(p, ky)];
+        const cx<T> zm = buf[Lds<T, NY>::at(p, (NY - ky) & (NY - 1))];
         store2(out + (long)ky * nx + x0 + 2 * p, cx<T>{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)},
                cx<T>{T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x)});
     }
@@ -615,16 +647,18 @@ __device__ __forceinline__ void r2c_pair_tile(const PRO &pro, int m, cx<T> *__re
             const int p = e / NYV, y = (e % NYV) * V;
             F a, b;
             pro(m, x0 + p, y, a, b);
-            cx<T> *d = buf + p * NY + y;
+            cx<T> *d = buf + Lds<T, NY>::at(p, y);
 #pragma unroll
             for (int v = 0; v < V; ++v) d[v] = {a[v], b[v]};
         }
     }
     lds_fft<T, LOGNY, NROW, NT, false>(buf, twy);
-    for (int e = threadIdx.x; e < NK * NROW; e += NT) {
-        const int p = e % NROW, ky = e / NROW;
-        const cx<T> zk = buf[p * NY + ky];
-        const cx<T> zm = buf[p * NY + ((NY - ky) & (NY - 1))];
+    constexpr int NO = ((NK + 15) / 16) * 16 * NROW;
+    for (int e = threadIdx.x; e < NO; e += NT) {
+        const int p = (e / 16) % NROW, ky = (e / (16 * NROW)) * 16 + e % 16;
+        if (ky >= NK) continue;
+        const cx<T> zk = buf[Lds<T, NY>::at(p, ky)];
+        const cx<T> zm = buf[Lds<T, NY>::at(p, (NY - ky) & (NY - 1))];
         oa[(long)ky * nx + x0 + p] = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
         ob[(long)ky * nx + x0 + p] = {T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x)};
     }
@@ -636,7 +670,7 @@ __global__ __launch_bounds__(NT) void k_yr2c(PRO pro, cx<T> *__restrict__ I, lon
                                              const cx<T> *__restrict__ twy)
 {
     constexpr int NY = 1 << LOGNY, NPR = TX / 2;
-    __shared__ cx<T> buf[NPR * NY];
+    __shared__ cx<T> buf[Lds<T, NY>::size(NPR)];
     const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TX;
     const int m = blockIdx.z;
     r2c_tile<T, LOGNY, NPR, PRO>(pro, m, I + m * sI + dst * fsI, x0, nx, buf, twy);
@@ -648,7 +682,7 @@ __global__ __launch_bounds__(NT) void k_yr2c_pair(PRO pro, cx<T> *__restrict__ I
                                                   int dstB, int nx, const cx<T> *__restrict__ twy)
 {
     constexpr int NY = 1 << LOGNY;
-    __shared__ cx<T> buf[TXP * NY];
+    __shared__ cx<T> buf[Lds<T, NY>::size(TXP)];
     const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TXP;
     const int m = blockIdx.z;
     r2c_pair_tile<T, LOGNY, TXP, PRO>(pro, m, I + m * sI + dstA * fsI, I + m * sI + dstB * fsI, x0, nx, buf, twy);
@@ -665,36 +699,41 @@ __global__ __launch_bounds__(NT) void k_xfwdinv(cx<T> *__restrict__ I, long sI, 
 {
     using G = XGeom<T, LOGNX>;
     static_assert(G::ROWS == ROWS, "x rows");
-    constexpr int NX = G::NX, NE = G::NE, PER = G::PER, NB = G::XNB;
-    __shared__ cx<T> W[NB * NE];
+    constexpr int NX = G::NX, PER = G::PER, NB = G::XNB;
+    using L = Lds<T, NX>;
+    __shared__ cx<T> W[L::size(NB * ROWS)];
     const int ky0 = blockIdx.x * ROWS;
-    cx<T> *base = I + blockIdx.z * sI;
+    using R = T;  // multiplier arithmetic: FP64 (FP64 form) or FP32 (FP32 form)
+    const int m = blockIdx.z;
+    cx<T> *base = I + m * sI;
     const cx<T> *in = base + src * fs;
+    const double *tht = kt.th + m * kt.ths;
     cx<T> s[PER];
-    double kxv[PER], kz[PER], th[PER];
+    double kxv[PER], thd[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int e = threadIdx.x + q * NT;
         const int kx = e % NX, ky = min(ky0 + e / NX, kt.nk - 1);
         s[q] = in[(long)ky * NX + kx];
         kxv[q] = kt.kx[kx];
+        if constexpr (sizeof(R) == 8) thd[q] = tht[(long)ky * NX + kx];
+        else thd[q] = 1.0;
     }
-    const double h = kt.h(blockIdx.z);
+    const double h = kt.h(m);
+    R kz[PER], th[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int e = threadIdx.x + q * NT;
         const int ky = ky0 + e / NX;
-        W[e] = (ky < kt.nk) ? s[q] : cx<T>{T(0), T(0)};
-        const double kyv = ky * kt.ky0;
-        kz[q] = sqrt(kxv[q] * kxv[q] + kyv * kyv);  // |k| and tanh(|k| h), FP64
-        th[q] = h > 0.0 ? tanh(kz[q] * h) : 1.0;
+        W[L::at(e / NX, e % NX)] = (ky < kt.nk) ? s[q] : cx<T>{T(0), T(0)};
+        kz_th<R>(kxv[q], ky * kt.ky0, h, thd[q], kz[q], th[q]);
     }
     lds_fft<T, LOGNX, ROWS, NT, false>(W, twx);
 #pragma unroll
-    for (int q = 0; q < PER; ++q) s[q] = W[threadIdx.x + q * NT];
-    double kj[PER];
+    for (int q = 0; q < PER; ++q) s[q] = W[L::at((threadIdx.x + q * NT) / NX, (threadIdx.x + q * NT) % NX)];
+    R kj[PER];
 #pragma unroll
-    for (int q = 0; q < PER; ++q) kj[q] = scale;
+    for (int q = 0; q < PER; ++q) kj[q] = R(scale);
     for (int j0 = 1; j0 <= ndst; j0 += NB) {
         __syncthreads();  // W has been read (forward result or the previous batch)
 #pragma unroll
@@ -704,8 +743,9 @@ __global__ __launch_bounds__(NT) void k_xfwdinv(cx<T> *__restrict__ I, long sI, 
 #pragma unroll
             for (int q = 0; q < PER; ++q) {
                 kj[q] *= kz[q];  // scale * |k|^j
-                const T mul = T((j & 1) ? kj[q] * th[q] : kj[q]);
-                W[b * NE + threadIdx.x + q * NT] = {mul * s[q].x, mul * s[q].y};
+                const T mul = T((j & 1) ? kj[q] * th[q] : kj[q]);  // odd j: tanh(|k| h)
+                const int e = threadIdx.x + q * NT;
+                W[L::at(b * ROWS + e / NX, e % NX)] = {mul * s[q].x, mul * s[q].y};
             }
         }
         lds_fft<T, LOGNX, ROWS * NB, NT, true>(W, twx);
@@ -717,7 +757,7 @@ __global__ __launch_bounds__(NT) void k_xfwdinv(cx<T> *__restrict__ I, long sI, 
             for (int q = 0; q < PER; ++q) {
                 const int e = threadIdx.x + q * NT;
                 const int kx = e % NX, ky = ky0 + e / NX;
-                if (ky < kt.nk) out[(long)ky * NX + kx] = W[b * NE + e];
+                if (ky < kt.nk) out[(long)ky * NX + kx] = W[L::at(b * ROWS + e / NX, kx)];
             }
         }
     }
@@ -756,7 +796,8 @@ __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, lo
     using G = XGeom<T, LOGNX>;
     static_assert(G::ROWS == ROWS, "x rows");
     constexpr int NX = G::NX, PER = G::PER;
-    __shared__ cx<T> S[2 * ROWS * NX];
+    using L = Lds<T, NX>;
+    __shared__ cx<T> S[L::size(2 * ROWS)];
     __shared__ double red[NT / 64];
     const int ky0 = blockIdx.x * ROWS;
     const int m = blockIdx.z;
@@ -775,8 +816,8 @@ __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, lo
         for (int q = 0; q < PER; ++q) {
             const int e = threadIdx.x + q * NT;
             const bool ok = ky0 + e / NX < kt.nk;
-            S[e] = ok ? va[q] : cx<T>{T(0), T(0)};
-            S[ROWS * NX + e] = ok ? vb[q] : cx<T>{T(0), T(0)};
+            S[L::at(e / NX, e % NX)] = ok ? va[q] : cx<T>{T(0), T(0)};
+            S[L::at(ROWS + e / NX, e % NX)] = ok ? vb[q] : cx<T>{T(0), T(0)};
         }
     }
     lds_fft<T, LOGNX, 2 * ROWS, NT, false>(S, twx);
@@ -796,7 +837,7 @@ __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, lo
         const int r = e / NX, kx = e % NX, ky = ky0 + r;
         if (ky >= kt.nk) continue;
         const long idx = (long)ky * NX + kx;
-        const cx<T> na = S[e], nb = S[ROWS * NX + e];
+        const cx<T> na = S[L::at(r, kx)], nb = S[L::at(ROWS + r, kx)];
         if (rk.mode == EPI_STORE_SPEC2) {
             cx<double> *k = rk.K + m * rk.sK;
             k[idx] = {scale * double(na.x), scale * double(na.y)};
@@ -805,7 +846,7 @@ __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, lo
         }
         const cx<double> ye = y[idx], yp = y[fsd + idx];
         const double kxv = kt.kx[kx], kyv = ky * kt.ky0;
-        const double w1 = dz_mul(sqrt(kxv * kxv + kyv * kyv), 1, kt.h(m));
+        const double w1 = sqrt(kxv * kxv + kyv * kyv) * kt.th[m * kt.ths + idx];  // |k| tanh(|k| h)
         // eta_t = W^(1) + R*(rest), phi_t = -g eta + R*(rest)  (hosm_2d_par.c:211-229)
         cx<double> ke = {w1 * yp.x + R * scale * double(na.x), w1 * yp.y + R * scale * double(na.y)};
         cx<double> kp = {-rk.g * ye.x + R * scale * double(nb.x), -rk.g * ye.y + R * scale * double(nb.y)};
@@ -873,151 +914,6 @@ __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, lo
             rk.hpart[m * gridDim.x + blockIdx.x] = s;
         }
     }
-}
-
-// ---------------------------------------------------------------------------
-// end of step: fixed-order sum of the Hamiltonian partials; t += dt.
-// grid (members), block NT
-__global__ __launch_bounds__(NT) void k_end_step(const double *__restrict__ hpart, int nparts, double *H,
-                                                 double *t, const double *dt, int advance)
-{
-    __shared__ double red[NT / 64];
-    const int m = blockIdx.x;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += NT) s += hpart[m * nparts + i];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double tot = 0.0;
-        for (int w = 0; w < NT / 64; ++w) tot += red[w];
-        H[m] = tot;
-        if (advance) t[m] += dt[m];
-    }
-}
-
-// generic Butcher combination for schemes with a full lower triangle:
-// out = u + dt * sum_p coef[p] K_p   (time_schemes_par.c:38-67, 98-116)
-struct CombArgs {
-    double coef[8];
-    int np;
-};
-__global__ __launch_bounds__(NT) void k_combine(const cx<double> *U, const cx<double> *__restrict__ K,
-                                                long sK_stage, long sM, cx<double> *out, long n, const double *dt,
-                                                CombArgs ca)
-{
-    const int m = blockIdx.y;
-    const double d = dt[m];
-    for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
-        cx<double> v = U[m * sM + i];
-        for (int p = 0; p < ca.np; ++p) {
-            const cx<double> k = K[p * sK_stage + m * sM + i];
-            v.x += d * ca.coef[p] * k.x;
-            v.y += d * ca.coef[p] * k.y;
-        }
-        out[m * sM + i] = v;
-    }
-}
-
-// pointwise spectral operators on FP64 fields [Nk][Nx]
-// (operators_par.c: Dx 50-71, Dy 105-116, Dz 168-192, Sum 266-276, Dealias 309-322)
-enum : int { SOP_DX = 1, SOP_DY = 2, SOP_DZ = 3, SOP_SUM = 4, SOP_DEALIAS = 5 };
-__global__ __launch_bounds__(NT) void k_spec_op(const cx<double> *__restrict__ a, const cx<double> *__restrict__ b,
-                                                cx<double> *__restrict__ out, int op, int nx, KTab kt)
-{
-    const long n = (long)kt.nk * nx;
-    for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
-        const int kx = (int)(i % nx), ky = (int)(i / nx);
-        const cx<double> s = a[i];
-        cx<double> r = s;
-        if (op == SOP_DX) {
-            const double k = kt.kx[kx];
-            r = {-k * s.y, k * s.x};
-        } else if (op == SOP_DY) {
-            const double k = ky * kt.ky0;
-            r = {-k * s.y, k * s.x};
-        } else if (op == SOP_DZ) {
-            const double kxv = kt.kx[kx], kyv = ky * kt.ky0;
-            const double m = dz_mul(sqrt(kxv * kxv + kyv * kyv), 1, kt.h(0));
-            r = {m * s.x, m * s.y};
-        } else if (op == SOP_SUM) {
-            r = {s.x + b[i].x, s.y + b[i].y};
-        } else if (op == SOP_DEALIAS) {
-            if (masked(kx, ky, nx, kt.mx, kt.my)) r = {0.0, 0.0};
-        }
-        out[i] = r;
-    }
-}
-
-// Hamiltonian of three FP64 spectral fields (hosm_2d_par.c:462-523):
-// partial sums per block, row j=0 weight 1, others 2.
-__global__ __launch_bounds__(NT) void k_hamiltonian(const cx<double> *__restrict__ eta,
-                                                    const cx<double> *__restrict__ etat,
-                                                    const cx<double> *__restrict__ phi, double g, int nx, int nk,
-                                                    double *part)
-{
-    __shared__ double red[NT / 64];
-    const long n = (long)nk * nx;
-    double h = 0.0;
-    for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
-        const double w = (i < nx) ? 1.0 : 2.0;
-        const cx<double> e = eta[i], t = etat[i], p = phi[i];
-        h += w * (g * (e.x * e.x + e.y * e.y) + p.x * t.x + p.y * t.y);
-    }
-    for (int off = 32; off > 0; off >>= 1) h += __shfl_down(h, off, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = h;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int w = 0; w < NT / 64; ++w) s += red[w];
-        part[blockIdx.x] = s;
-    }
-}
-
-// product of two physical fields (the pointwise step of Mult, operators_par.c:222-231)
-template <class T>
-struct ProMul {
-    Phys<T> ph;
-    int fa, fb;
-    __device__ __forceinline__ vec_t<T> operator()(int m, int x, int y) const
-    {
-        return ph.vat(m, fa, x, y) * ph.vat(m, fb, x, y);
-    }
-};
-
-// FP64 spectral field [Nk][Nx] (row stride nx) copy, optionally from a
-// different source; used for state set-up on the device.
-__global__ __launch_bounds__(NT) void k_copy_c64(const cx<double> *__restrict__ src, cx<double> *__restrict__ dst,
-                                                 long n)
-{
-    // four 16-byte loads in flight per thread per iteration
-    constexpr int U = 4;
-    const long step = (long)gridDim.x * NT * U;
-    for (long i0 = blockIdx.x * (long)NT * U + threadIdx.x; i0 < n; i0 += step) {
-        cx<double> v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const long i = i0 + (long)u * NT;
-            if (i < n) v[u] = src[i];
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const long i = i0 + (long)u * NT;
-            if (i < n) dst[i] = v[u];
-        }
-    }
-}
-
-// real [Nx][Ny] (T) <-> padded double host layout helpers
-template <class T>
-__global__ __launch_bounds__(NT) void k_real_from_f64(const double *__restrict__ src, T *__restrict__ dst, long n)
-{
-    for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) dst[i] = T(src[i]);
-}
-template <class T>
-__global__ __launch_bounds__(NT) void k_real_to_f64(const T *__restrict__ src, double *__restrict__ dst, long n)
-{
-    for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) dst[i] = double(src[i]);
 }
 
 }  // namespace hos

@@ -63,6 +63,21 @@ def test_rhs_quirk_ramp(hos, oracle_mod, quirk, ramp):
         assert rel_l2(s.rhs(u, t), o.rhs_hos(u, t)) < TOL64
 
 
+@pytest.mark.parametrize("Nx,Ny", [(2048, 32), (32, 2048)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_rhs_2048_rows_vs_oracle(hos, oracle_mod, Nx, Ny, prec):
+    """Rows of 2048 points in x and in y, finite depth: the largest FFT passes
+    (the FP64 ones hold more than 64 KiB of LDS per workgroup)."""
+    nlevs, depth = 2, 0.9
+    Lx, Ly = 2 * np.pi * Nx / 32, 2 * np.pi * Ny / 32
+    print(f"rhs {Nx}x{Ny} prec={prec}", flush=True)
+    o = oracle_mod.Oracle(Nx, Ny, nlevs, Lx, Ly, depth=depth)
+    s = make(hos, Nx, Ny, nlevs, prec, Lx=Lx, Ly=Ly, depth=depth)
+    u = random_state(Nx, Ny, nlevs, amp=0.05, slope=3.0, seed=11)
+    err = rel_l2(s.rhs(u), o.rhs_hos(u))
+    assert err < (TOL64 if prec == 0 else TOL32), err
+
+
 @pytest.mark.parametrize("nlevs", [1, 2, 4])
 def test_rhs_fp32_vs_oracle(hos, oracle_mod, nlevs):
     Nx, Ny = 128, 64
