@@ -56,6 +56,7 @@ def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_s
         s.set_state(u0, m)
         s.set_member(m, w["depth"], dt)
     tiles = s.autotune(3) if autotune else None
+    xpass = s.get_tile(hos.HOS_TILE_XPASS)
     s.use_graph(graph)
     s.step(warmup, dt)
     s.sync()
@@ -75,6 +76,7 @@ def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_s
         "state": s.get_state(0),
         "bytes_per_pt_step": hos.bytes_per_point_step(p),
         "tiles": tiles,
+        "xpass": "row per wave" if xpass == 2 else "rows per workgroup",
     }
     if profile_steps:
         # same workload, stream-ordered launches with a HIP event pair around
@@ -253,6 +255,7 @@ def main():
                         "wavelengths, kp*h=1, M=5, JONSWAP gamma=5 kp*Hs/2=0.12 seed 1, RK4 dt=Tp/100, all "
                         "members in the same launches; compared with the plain FP64 single instance",
             "members": args.members, "graph": graph, "y_tiles_c2r_r2c_pair": f32["tiles"],
+            "x_inverse_passes": f32["xpass"],
             "form": "FP32 FFTs/products with Kahan order sums; FP64 state, RK combination, linear terms",
             "parallelism": "single GPU",
         },

@@ -43,8 +43,9 @@ EXPORTS = [
     "hos_last_error", "hos_device_name", "hos_num_members", "hos_bytes_per_point_step", "hos_step_times",
     "hos_profile", "hos_kernel_stats", "hos_spectral_op", "hos_mult", "hos_hamiltonian",
     "hos_set_active", "hos_get_active", "hos_set_member", "hos_copy_member", "hos_step_async", "hos_wait",
-    "hos_autotune", "hos_copy_bandwidth",
+    "hos_autotune", "hos_copy_bandwidth", "hos_get_tile",
 ]
+HOS_TILE_C2R, HOS_TILE_R2C, HOS_TILE_PAIR, HOS_TILE_ALL, HOS_TILE_XPASS = 0, 1, 2, 3, 4
 HOS_OP_DX, HOS_OP_DY, HOS_OP_DZ, HOS_OP_SUM, HOS_OP_DEALIAS = 1, 2, 3, 4, 5
 # include/hos_ic.h
 IC_EXPORTS = ["hos_ic_jonswap", "hos_ic_uniform", "hos_ic_peak_period", "hos_dealias_state"]
@@ -97,6 +98,7 @@ def load():
     lib.hos_last_step_ms.argtypes = [vp, dp]
     lib.hos_use_graph.argtypes = [vp, ip]
     lib.hos_set_tile.argtypes = [vp, ip, ip]
+    lib.hos_get_tile.argtypes = [vp, ip]
     lib.hos_last_error.restype = ctypes.c_char_p
     lib.hos_device_name.argtypes = [ctypes.c_char_p, ip]
     lib.hos_num_members.argtypes = [vp]
@@ -420,15 +422,23 @@ class Solver:
         _check(self.lib.hos_use_graph(self.h, int(on)))
 
     def autotune(self, nsteps: int = 2):
-        """hos_autotune: returns the chosen (c2r, r2c, pair) y tiles."""
+        """hos_autotune: returns the chosen (c2r, r2c, pair) y tiles; the
+        x-pass mode it chose is get_tile(HOS_TILE_XPASS)."""
         t = (ctypes.c_int * 3)()
         _check(self.lib.hos_autotune(self.h, nsteps, t))
         return tuple(t)
 
     def set_tile(self, tx: int, kind: int = 3):
         """y-pass tile (4, 8, 16; 0 = default) for kind HOS_TILE_C2R (0),
-        HOS_TILE_R2C (1), HOS_TILE_PAIR (2) or all (3)."""
+        HOS_TILE_R2C (1), HOS_TILE_PAIR (2) or all (3); for HOS_TILE_XPASS (4)
+        the x-pass mode (1 rows per workgroup, 2 row per wave, 0 default)."""
         _check(self.lib.hos_set_tile(self.h, kind, tx))
+
+    def get_tile(self, kind: int) -> int:
+        v = self.lib.hos_get_tile(self.h, kind)
+        if v < 0:
+            raise HosError(self.lib.hos_last_error().decode())
+        return v
 
     def sync(self):
         _check(self.lib.hos_sync(self.h))

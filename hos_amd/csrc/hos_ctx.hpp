@@ -29,6 +29,7 @@ struct hos_ctx {
     int nm = 0, M = 0, L = 0, Nk = 0, lognx = 0, logny = 0, mx = 0, my = 0;
     int nI = 0, nP = 0;
     int tx[3] = {8, 4, 4};  // y tiles: c2r, r2c, pair
+    int xwave = 0;          // x inverse passes one row per wave (where available)
     bool f32 = false;
     long fs = 0;    // complex elements per spectral field  (Nk*Nx)
     long fsP = 0;   // real elements per physical field     (Nx*Ny)
@@ -159,6 +160,22 @@ constexpr int tile_lds_max(int tx) { return tx == 4 ? 163840 : 65536; }
     default: return fail(HOS_E_INVALID, "unsupported tile %d", tx);                                \
     }
 
+// x inverse passes (k_xinv, k_xfwdinv): one row per wave (k_*_w) is
+// available in the FP32 form for Nx = 512 and 1024 (at 2048 a wave's row
+// needs more registers than a lane has); c->xwave selects it.
+template <class T>
+constexpr bool x_wave(int lognx)
+{
+    return sizeof(T) == 4 && (lognx == 9 || lognx == 10);
+}
+inline bool x_wave_ok(const hos_ctx *c) { return c->f32 ? x_wave<float>(c->lognx) : x_wave<double>(c->lognx); }
+// stage-0 Hamiltonian partials per member written by the final x pass
+inline int x_nparts(const hos_ctx *c)
+{
+    const int rows = c->lognx >= 10 ? 1 : (1 << (10 - c->lognx));
+    return (c->Nk + rows - 1) / rows;
+}
+
 template <class T>
 struct Eng {
     hos_ctx *c;
@@ -204,8 +221,19 @@ struct Eng {
             // FP64 form: the tanh(|k| h) table is read once per group
             const double thb = sizeof(T) == 8 ? 8.0 * ngroups : 0.0;
             Prof pf(c, HOS_K_XINV, (double)nm * c->fs * (16.0 * ngroups + thb + 2.0 * sizeof(T) * nops));
-            hipLaunchKernelGGL((k_xinv<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, Y, sY(), I(), sI(), c->fs, ops,
-                               ktab(c), twx());
+            bool waved = false;
+            if constexpr (x_wave<T>(LG)) {
+                if (c->xwave) {
+                    dim3 gw((c->Nk + XW - 1) / XW, ngroups, nm);
+                    hipLaunchKernelGGL((k_xinv_w<T, LG>), gw, dim3(NT), 0, c->stream, Y, sY(), I(), sI(), c->fs, ops,
+                                       ktab(c), twx());
+                    waved = true;
+                }
+            }
+            if (!waved) {
+                hipLaunchKernelGGL((k_xinv<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, Y, sY(), I(), sI(), c->fs,
+                                   ops, ktab(c), twx());
+            }
         });
         return HOS_OK;
     }
@@ -258,8 +286,19 @@ struct Eng {
             constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
             dim3 grid((c->Nk + ROWS - 1) / ROWS, 1, nm);
             Prof pf(c, HOS_K_XFWDINV, (double)nm * c->fs * (2.0 * sizeof(T) * (1 + ndst) + (sizeof(T) == 8 ? 8.0 : 0.0)));
-            hipLaunchKernelGGL((k_xfwdinv<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, I(), sI(), c->fs, src, dst0,
-                               ndst, scale, ktab(c), twx());
+            bool waved = false;
+            if constexpr (x_wave<T>(LG)) {
+                if (c->xwave) {
+                    dim3 gw((c->Nk + XW - 1) / XW, 1, nm);
+                    hipLaunchKernelGGL((k_xfwdinv_w<T, LG>), gw, dim3(NT), 0, c->stream, I(), sI(), c->fs, src, dst0,
+                                       ndst, scale, ktab(c), twx());
+                    waved = true;
+                }
+            }
+            if (!waved) {
+                hipLaunchKernelGGL((k_xfwdinv<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, I(), sI(), c->fs, src, dst0,
+                                   ndst, scale, ktab(c), twx());
+            }
         });
         return HOS_OK;
     }

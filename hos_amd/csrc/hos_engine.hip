@@ -145,6 +145,11 @@ static int default_tx(bool f32, int logny, int kind)
     return tx;
 }
 
+// x inverse passes one row per wave by default where available and the
+// batch is large enough to fill the chip with waves (measured: one 1024^2
+// member runs faster with rows per workgroup)
+static int default_xwave(const hos_ctx *c) { return x_wave_ok(c) && c->nm >= 4 ? 1 : 0; }
+
 static int validate(const hos_params *p, int nm)
 {
     if (!p) return fail(HOS_E_INVALID, "null params");
@@ -249,7 +254,6 @@ static void release(hos_ctx *c)
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
-static int xrows(int lognx) { return lognx >= 10 ? 1 : (1 << (10 - lognx)); }
 
 extern "C" int hos_create(const hos_params *p, int nmembers, hos_ctx **out)
 {
@@ -267,6 +271,7 @@ extern "C" int hos_create(const hos_params *p, int nmembers, hos_ctx **out)
     }
     c->nK = c->subdiag ? 2 : (c->S > 2 ? c->S : 2);
     for (int k = HOS_TILE_C2R; k <= HOS_TILE_PAIR; ++k) c->tx[k] = default_tx(c->f32, c->logny, k);
+    c->xwave = default_xwave(c);
     const int nm = nmembers;
     auto bail = [&](int code) {
         release(c);
@@ -281,7 +286,7 @@ extern "C" int hos_create(const hos_params *p, int nmembers, hos_ctx **out)
     const size_t st = (size_t)nm * 2 * c->fs * sizeof(cx<double>);
     const size_t kslots = (size_t)(c->subdiag ? 2 : c->S * nm);
     const size_t kbytes = (kslots < 2 ? 2 : kslots) * 2 * c->fs * sizeof(cx<double>);
-    const int nparts = (c->Nk + xrows(c->lognx) - 1) / xrows(c->lognx);
+    const int nparts = x_nparts(c);
     struct {
         void **p;
         size_t n;
@@ -377,7 +382,7 @@ template <class T>
 static int launch_step(hos_ctx *c)
 {
     Eng<T> e{c, c->active};
-    const int nparts = (c->Nk + xrows(c->lognx) - 1) / xrows(c->lognx);
+    const int nparts = x_nparts(c);
     int rc = HOS_OK;
     for (int s = 0; s < c->S && !rc; ++s) {
         RkArgs rk = rk_base(c);
@@ -640,6 +645,14 @@ extern "C" int hos_sync(hos_ctx *c)
 extern "C" int hos_set_tile(hos_ctx *c, int kind, int tx)
 {
     if (!c) return fail(HOS_E_INVALID, "null context");
+    if (kind == HOS_TILE_XPASS) {
+        if (tx < 0 || tx > 2) return fail(HOS_E_INVALID, "x-pass mode %d: 0 default, 1 rows per workgroup, 2 row per wave", tx);
+        if (tx == 2 && !x_wave_ok(c)) return fail(HOS_E_INVALID, "row-per-wave x passes need the FP32 form and Nx of 512 or 1024");
+        if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
+        c->xwave = tx == 0 ? default_xwave(c) : tx - 1;
+        drop_graphs(c);
+        return HOS_OK;
+    }
     if (kind < HOS_TILE_C2R || kind > HOS_TILE_ALL) return fail(HOS_E_INVALID, "unknown tile kind %d", kind);
     for (int k = HOS_TILE_C2R; k <= HOS_TILE_PAIR; ++k) {
         if (kind != HOS_TILE_ALL && kind != k) continue;
@@ -655,10 +668,19 @@ extern "C" int hos_set_tile(hos_ctx *c, int kind, int tx)
     return HOS_OK;
 }
 
+extern "C" int hos_get_tile(hos_ctx *c, int kind)
+{
+    if (!c) return fail(HOS_E_INVALID, "null context");
+    if (kind == HOS_TILE_XPASS) return c->xwave ? 2 : 1;
+    if (kind < HOS_TILE_C2R || kind > HOS_TILE_PAIR) return fail(HOS_E_INVALID, "unknown tile kind %d", kind);
+    return c->tx[kind];
+}
+
 // ---------------------------------------------------------------------------
-// autotuner: y tiles per kernel kind, timed on the context's own members
+// autotuner: y tiles per kernel kind, then the x-pass mode, timed on the
+// context's own members
 static std::mutex g_tune_mu;
-static std::map<std::tuple<int, int, int, int, int, int>, std::array<int, 3>> g_tune_cache;
+static std::map<std::tuple<int, int, int, int, int, int>, std::array<int, 4>> g_tune_cache;
 
 extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
 {
@@ -671,6 +693,7 @@ extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
         auto it = g_tune_cache.find(key);
         if (it != g_tune_cache.end()) {
             for (int k = 0; k < 3; ++k) c->tx[k] = it->second[k];
+            c->xwave = it->second[3];
             drop_graphs(c);
             if (tiles_out)
                 for (int k = 0; k < 3; ++k) tiles_out[k] = c->tx[k];
@@ -692,10 +715,20 @@ extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
     for (int m = 0; m < c->nm; ++m) dtev[m] = c->host_dt[m] > 0 ? c->host_dt[m] : 1e-3;
     int rc = upload_dt(c, dtev.data());
     const int cand[3] = {4, 8, 16};
-    std::array<int, 3> best = {c->tx[0], c->tx[1], c->tx[2]};
+    std::array<int, 4> best = {c->tx[0], c->tx[1], c->tx[2], c->xwave};
     double best_ms = 1e30;
     const int saved_graph = c->use_graph;
     c->use_graph = 1;
+    // one timed trial of the current launch shapes
+    auto trial = [&](double &ms) {
+        drop_graphs(c);
+        int r = step_enqueue(c, 1);  // capture + warm
+        if (!r) r = step_finish(c);
+        if (!r) r = step_enqueue(c, nsteps);
+        if (!r) r = step_finish(c);
+        ms = c->last_ms;
+        return r;
+    };
     for (int a = 0; a < 3 && !rc; ++a)
         for (int b = 0; b < 3 && !rc; ++b)
             for (int d = 0; d < 3 && !rc; ++d) {
@@ -704,18 +737,27 @@ extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
                 for (int k = 0; k < 3; ++k) ok &= tx_fits(c->f32, c->logny, tt[k]) && c->p.Nx % tt[k] == 0;
                 if (!ok) continue;
                 for (int k = 0; k < 3; ++k) c->tx[k] = tt[k];
-                drop_graphs(c);
-                rc = step_enqueue(c, 1);  // capture + warm
-                if (!rc) rc = step_finish(c);
-                if (!rc) rc = step_enqueue(c, nsteps);
-                if (!rc) rc = step_finish(c);
-                if (!rc && c->last_ms < best_ms) {
-                    best_ms = c->last_ms;
-                    best = {tt[0], tt[1], tt[2]};
+                double ms = 0.0;
+                rc = trial(ms);
+                if (!rc && ms < best_ms) {
+                    best_ms = ms;
+                    best = {tt[0], tt[1], tt[2], c->xwave};
                 }
             }
+    if (!rc && x_wave_ok(c)) {
+        // the other x-pass mode with the best tiles
+        for (int k = 0; k < 3; ++k) c->tx[k] = best[k];
+        c->xwave = !best[3];
+        double ms = 0.0;
+        rc = trial(ms);
+        if (!rc && ms < best_ms) {
+            best_ms = ms;
+            best[3] = c->xwave;
+        }
+    }
     c->use_graph = saved_graph;
     for (int k = 0; k < 3; ++k) c->tx[k] = best[k];
+    c->xwave = best[3];
     drop_graphs(c);
     (void)hipMemcpyAsync(c->U, saveU, st, hipMemcpyDeviceToDevice, c->stream);
     (void)hipMemcpyAsync(c->Y, saveY, st, hipMemcpyDeviceToDevice, c->stream);
@@ -987,7 +1029,7 @@ extern "C" int hos_hamiltonian(hos_ctx *c, const double *heta, const double *het
     HIP_TRY(hipMemcpyAsync(c->K, heta, n, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->K + c->fs, heta_t, n, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->K + 2 * c->fs, hphi, n, hipMemcpyHostToDevice, c->stream));
-    const int nparts = (c->Nk + xrows(c->lognx) - 1) / xrows(c->lognx) * c->nm;
+    const int nparts = x_nparts(c) * c->nm;
     const int grid = std::min(nparts, 64);
     hipLaunchKernelGGL(k_hamiltonian, dim3(grid), dim3(NT), 0, c->stream, c->K, c->K + c->fs, c->K + 2 * c->fs,
                        c->p.g, c->p.Nx, c->Nk, c->hpart);

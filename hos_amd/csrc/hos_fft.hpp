@@ -155,6 +155,45 @@ __device__ __forceinline__ void dft_small(cx<T> *v)
     }
 }
 
+// Stockham twiddles of butterfly j of a stage (NS = product of the earlier
+// radices): input r is multiplied by w^r, w = exp(-+2 pi i (j mod NS)/(NS R)).
+// Powers below 4 by repeated products of w, higher ones as (w^4)^q * w^b with
+// w^4 a second table entry, so no power is more than three products away from
+// a table value.
+template <class T, int R, int N, int NS, bool INV>
+__device__ __forceinline__ void stage_twiddles(cx<T> *v, int j, const cx<T> *__restrict__ tw)
+{
+    if constexpr (NS > 1) {
+        const int k = j % NS;
+        constexpr int STRIDE = N / (NS * R);
+        cx<T> w1 = tw[k * STRIDE];
+        if (INV) w1.y = -w1.y;
+        cx<T> wb[4];
+        wb[0] = {T(1), T(0)};
+        wb[1] = w1;
+        if constexpr (R > 2) {
+            wb[2] = cmul(w1, w1);
+            wb[3] = cmul(wb[2], w1);
+        }
+        cx<T> wq[4];
+        wq[0] = {T(1), T(0)};
+        if constexpr (R > 4) {
+            cx<T> w4 = tw[4 * k * STRIDE];
+            if (INV) w4.y = -w4.y;
+            wq[1] = w4;
+            if constexpr (R > 8) {
+                wq[2] = cmul(w4, w4);
+                wq[3] = cmul(wq[2], w4);
+            }
+        }
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            const cx<T> w = (r >> 2) == 0 ? wb[r & 3] : ((r & 3) == 0 ? wq[r >> 2] : cmul(wq[r >> 2], wb[r & 3]));
+            v[r] = cmul(v[r], w);
+        }
+    }
+}
+
 template <class T, int R, int LOGN, int NROWS, int NT, bool INV, int NS>
 __device__ __forceinline__ void fft_stage(cx<T> *buf, const cx<T> *__restrict__ tw)
 {
@@ -175,39 +214,7 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, const cx<T> *__restrict__ 
             const int rb = row * L::PITCH + j + (j >> 4);
 #pragma unroll
             for (int r = 0; r < R; ++r) v[q][r] = buf[rb + r * NR + ((r * NR) >> 4)];
-            if constexpr (NS > 1) {
-                // w^r, r = 1..R-1, w = exp(-+2 pi i k/(NS R)): powers below 4 by
-                // repeated products of w, higher ones as (w^4)^q * w^b with w^4 a
-                // second table entry, so no power is more than three products
-                // away from a table value.
-                const int k = j % NS;
-                constexpr int STRIDE = N / (NS * R);
-                cx<T> w1 = tw[k * STRIDE];
-                if (INV) w1.y = -w1.y;
-                cx<T> wb[4];
-                wb[0] = {T(1), T(0)};
-                wb[1] = w1;
-                if constexpr (R > 2) {
-                    wb[2] = cmul(w1, w1);
-                    wb[3] = cmul(wb[2], w1);
-                }
-                cx<T> wq[4];
-                wq[0] = {T(1), T(0)};
-                if constexpr (R > 4) {
-                    cx<T> w4 = tw[4 * k * STRIDE];
-                    if (INV) w4.y = -w4.y;
-                    wq[1] = w4;
-                    if constexpr (R > 8) {
-                        wq[2] = cmul(w4, w4);
-                        wq[3] = cmul(wq[2], w4);
-                    }
-                }
-#pragma unroll
-                for (int r = 1; r < R; ++r) {
-                    const cx<T> w = (r >> 2) == 0 ? wb[r & 3] : ((r & 3) == 0 ? wq[r >> 2] : cmul(wq[r >> 2], wb[r & 3]));
-                    v[q][r] = cmul(v[q][r], w);
-                }
-            }
+            stage_twiddles<T, R, N, NS, INV>(v[q], j, tw);
             dft_small<T, R, INV>(v[q]);
         }
     }
@@ -246,6 +253,91 @@ __device__ __forceinline__ void lds_fft(cx<T> *buf, const cx<T> *__restrict__ tw
 {
     __syncthreads();
     fft_stages<T, LOGN, NROWS, NT, INV>(buf, tw);
+}
+
+// ---------------------------------------------------------------------------
+// Wave FFT: one 64-lane wave transforms one row of N = 64 E complex values
+// held in registers, lane l owning elements l + 64 m (m = 0..E-1), in natural
+// order on input and on output.  Stockham stages of radix 16/8/4 in which
+// every stage has N/R >= 64 butterflies (one or more per lane): the first
+// stage reads its inputs straight from the registers (butterfly j = l + 64 q
+// needs elements j + r N/R, all owned by lane l), the last one writes its
+// outputs back to them (Stockham's last stage stores element j + r N/R), and
+// the stages in between exchange through the wave's own padded LDS row.  Only
+// the wave's own lanes touch that row and a wave's LDS operations complete in
+// issue order, so no workgroup barrier is needed, only compiler ordering.
+__host__ __device__ constexpr int wave_stage_bits(int logn, int s)
+{
+    // 512 = 8 8 8, 1024 = 16 16 4, 2048 = 16 16 8
+    return logn == 9 ? (s < 3 ? 3 : 0)
+         : logn == 10 ? (s < 2 ? 4 : (s == 2 ? 2 : 0))
+         : logn == 11 ? (s < 2 ? 4 : (s == 2 ? 3 : 0)) : 0;
+}
+__host__ __device__ constexpr int wave_nstages(int logn)
+{
+    int n = 0;
+    while (wave_stage_bits(logn, n) > 0) ++n;
+    return n;
+}
+
+// orders the LDS accesses of the wave's lanes on either side (no instruction
+// is needed: the wave's LDS operations complete in order)
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <class T, int LOGN, bool INV, int S, int NS>
+__device__ __forceinline__ void wave_stage(cx<T> *v, cx<T> *row, const cx<T> *__restrict__ tw, int lane)
+{
+    constexpr int N = 1 << LOGN, R = 1 << wave_stage_bits(LOGN, S), NR = N / R, B = NR / 64;
+    constexpr bool FIRST = S == 0, LAST = S + 1 == wave_nstages(LOGN);
+    static_assert(NR % 64 == 0, "a butterfly per lane at least");
+    cx<T> u[B][R];
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+        const int j = lane + 64 * q;
+        const int rb = j + (j >> 4);  // Lds<T, N>::at(0, j + r NR), as in fft_stage
+#pragma unroll
+        for (int r = 0; r < R; ++r) u[q][r] = FIRST ? v[q + r * B] : row[rb + r * NR + ((r * NR) >> 4)];
+    }
+    if constexpr (!FIRST) wave_sync();  // every lane has read before any lane writes
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+        const int j = lane + 64 * q;
+        stage_twiddles<T, R, N, NS, INV>(u[q], j, tw);
+        dft_small<T, R, INV>(u[q]);
+        if constexpr (LAST) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q + r * B] = u[q][r];
+        } else {
+            const int idxD = (j / NS) * NS * R + (j % NS);
+            const int wb = idxD + (idxD >> 4);
+#pragma unroll
+            for (int r = 0; r < R; ++r) row[wb + r * NS + ((r * NS) >> 4)] = u[q][r];
+        }
+    }
+    if constexpr (!LAST) wave_sync();  // the writes precede the next stage's reads
+}
+
+template <class T, int LOGN, bool INV, int S = 0, int NS = 1>
+__device__ __forceinline__ void wave_stages(cx<T> *v, cx<T> *row, const cx<T> *__restrict__ tw, int lane)
+{
+    if constexpr (S < wave_nstages(LOGN)) {
+        wave_stage<T, LOGN, INV, S, NS>(v, row, tw, lane);
+        wave_stages<T, LOGN, INV, S + 1, NS * (1 << wave_stage_bits(LOGN, S))>(v, row, tw, lane);
+    }
+}
+
+// unnormalised FFT of the wave's row v[E] (see above); row is the wave's LDS
+// row of Lds<T, N>::PITCH elements
+template <class T, int LOGN, bool INV>
+__device__ __forceinline__ void wave_fft(cx<T> *v, cx<T> *row, const cx<T> *__restrict__ tw, int lane)
+{
+    static_assert(wave_nstages(LOGN) > 0, "wave FFT lengths: 512, 1024, 2048");
+    wave_stages<T, LOGN, INV>(v, row, tw, lane);
 }
 
 }  // namespace hos

@@ -788,6 +788,81 @@ struct RkArgs {
     const double *mean0;  // [members]
 };
 
+// RHS at spectral point idx = (ky, kx) of member m from the forward-transformed
+// nonlinear terms na (eta_t) and nb (phi_t) and the RK stage update
+// (time_schemes_par.c:38-67 / sol_update_RK), FP64.  hsum accumulates the
+// stage-0 Hamiltonian terms of the point.
+template <class T>
+__device__ __forceinline__ void rk_epilogue(const RkArgs &rk, const KTab &kt, int m, long idx, int kx, int ky, int NX,
+                                            cx<T> na, cx<T> nb, double scale, double R, double dt,
+                                            const cx<double> *y, long fsd, double &hsum)
+{
+    if (rk.mode == EPI_STORE_SPEC2) {
+        cx<double> *k = rk.K + m * rk.sK;
+        k[idx] = {scale * double(na.x), scale * double(na.y)};
+        k[fsd + idx] = {scale * double(nb.x), scale * double(nb.y)};
+        return;
+    }
+    const cx<double> ye = y[idx], yp = y[fsd + idx];
+    const double kxv = kt.kx[kx], kyv = ky * kt.ky0;
+    const double w1 = sqrt(kxv * kxv + kyv * kyv) * kt.th[m * kt.ths + idx];  // |k| tanh(|k| h)
+    // eta_t = W^(1) + R*(rest), phi_t = -g eta + R*(rest)  (hosm_2d_par.c:211-229)
+    cx<double> ke = {w1 * yp.x + R * scale * double(na.x), w1 * yp.y + R * scale * double(na.y)};
+    cx<double> kp = {-rk.g * ye.x + R * scale * double(nb.x), -rk.g * ye.y + R * scale * double(nb.y)};
+    if (masked(kx, ky, NX, kt.mx, kt.my)) {
+        ke = {0.0, 0.0};
+        kp = {0.0, 0.0};
+    }
+    if (rk.mode == EPI_STORE_K) {
+        cx<double> *k = rk.K + m * rk.sK;
+        k[idx] = ke;
+        k[fsd + idx] = kp;
+        if (rk.stage == 0 && rk.hpart) {
+            const cx<double> *u = rk.U + m * 2 * fsd;
+            const cx<double> ue = u[idx], up = u[fsd + idx];
+            const double w = (ky == 0) ? 1.0 : 2.0;
+            hsum += w * (rk.g * (ue.x * ue.x + ue.y * ue.y) + up.x * ke.x + up.y * ke.y);
+            if (idx == 0) rk.vol[m] = ue.x;
+        }
+        return;
+    }
+    // EPI_RK_SUBDIAG: Y_{s+1} = u + dt b[s+1][s] K_s ; acc += dt c[s] K_s
+    cx<double> *u = rk.U + m * 2 * fsd;
+    cx<double> *acc = rk.A + m * 2 * fsd;
+    cx<double> *yn = rk.Yn + m * 2 * fsd;
+    const cx<double> ue = u[idx], up = u[fsd + idx];
+    if (rk.stage == 0 && rk.hpart) {
+        // Hamiltonian, hosm_2d_par.c:462-523: weight 1 on row j=0, 2 elsewhere
+        const double w = (ky == 0) ? 1.0 : 2.0;
+        hsum += w * (rk.g * (ue.x * ue.x + ue.y * ue.y) + up.x * ke.x + up.y * ke.y);
+        if (idx == 0) rk.vol[m] = ue.x;
+    }
+    const double fc = dt * rk.c_s;
+    cx<double> ae, ap;
+    if (rk.stage == 0) {
+        ae = ue;
+        ap = up;
+    } else {
+        ae = acc[idx];
+        ap = acc[fsd + idx];
+    }
+    ae = {ae.x + fc * ke.x, ae.y + fc * ke.y};
+    ap = {ap.x + fc * kp.x, ap.y + fc * kp.y};
+    if (rk.stage == rk.nstages - 1) {
+        if (rk.mean_level && idx == 0) ae = {rk.mean0[m], 0.0};
+        u[idx] = ae;
+        u[fsd + idx] = ap;
+        yn[idx] = ae;
+        yn[fsd + idx] = ap;
+    } else {
+        acc[idx] = ae;
+        acc[fsd + idx] = ap;
+        const double fb = dt * rk.b_next;
+        yn[idx] = {ue.x + fb * ke.x, ue.y + fb * ke.y};
+        yn[fsd + idx] = {up.x + fb * kp.x, up.y + fb * kp.y};
+    }
+}
+
 template <class T, int LOGNX, int ROWS>
 __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, long sI, long fs, int iA, int iB,
                                                  const cx<double> *Y, long sY, double scale, KTab kt,
@@ -837,71 +912,7 @@ __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, lo
         const int r = e / NX, kx = e % NX, ky = ky0 + r;
         if (ky >= kt.nk) continue;
         const long idx = (long)ky * NX + kx;
-        const cx<T> na = S[L::at(r, kx)], nb = S[L::at(ROWS + r, kx)];
-        if (rk.mode == EPI_STORE_SPEC2) {
-            cx<double> *k = rk.K + m * rk.sK;
-            k[idx] = {scale * double(na.x), scale * double(na.y)};
-            k[fsd + idx] = {scale * double(nb.x), scale * double(nb.y)};
-            continue;
-        }
-        const cx<double> ye = y[idx], yp = y[fsd + idx];
-        const double kxv = kt.kx[kx], kyv = ky * kt.ky0;
-        const double w1 = sqrt(kxv * kxv + kyv * kyv) * kt.th[m * kt.ths + idx];  // |k| tanh(|k| h)
-        // eta_t = W^(1) + R*(rest), phi_t = -g eta + R*(rest)  (hosm_2d_par.c:211-229)
-        cx<double> ke = {w1 * yp.x + R * scale * double(na.x), w1 * yp.y + R * scale * double(na.y)};
-        cx<double> kp = {-rk.g * ye.x + R * scale * double(nb.x), -rk.g * ye.y + R * scale * double(nb.y)};
-        if (masked(kx, ky, NX, kt.mx, kt.my)) {
-            ke = {0.0, 0.0};
-            kp = {0.0, 0.0};
-        }
-        if (rk.mode == EPI_STORE_K) {
-            cx<double> *k = rk.K + m * rk.sK;
-            k[idx] = ke;
-            k[fsd + idx] = kp;
-            if (rk.stage == 0 && rk.hpart) {
-                const cx<double> *u = rk.U + m * 2 * fsd;
-                const cx<double> ue = u[idx], up = u[fsd + idx];
-                const double w = (ky == 0) ? 1.0 : 2.0;
-                hsum += w * (rk.g * (ue.x * ue.x + ue.y * ue.y) + up.x * ke.x + up.y * ke.y);
-                if (idx == 0) rk.vol[m] = ue.x;
-            }
-            continue;
-        }
-        // EPI_RK_SUBDIAG: Y_{s+1} = u + dt b[s+1][s] K_s ; acc += dt c[s] K_s
-        cx<double> *u = rk.U + m * 2 * fsd;
-        cx<double> *acc = rk.A + m * 2 * fsd;
-        cx<double> *yn = rk.Yn + m * 2 * fsd;
-        const cx<double> ue = u[idx], up = u[fsd + idx];
-        if (rk.stage == 0 && rk.hpart) {
-            // Hamiltonian, hosm_2d_par.c:462-523: weight 1 on row j=0, 2 elsewhere
-            const double w = (ky == 0) ? 1.0 : 2.0;
-            hsum += w * (rk.g * (ue.x * ue.x + ue.y * ue.y) + up.x * ke.x + up.y * ke.y);
-            if (idx == 0) rk.vol[m] = ue.x;
-        }
-        const double fc = dt * rk.c_s;
-        cx<double> ae, ap;
-        if (rk.stage == 0) {
-            ae = ue;
-            ap = up;
-        } else {
-            ae = acc[idx];
-            ap = acc[fsd + idx];
-        }
-        ae = {ae.x + fc * ke.x, ae.y + fc * ke.y};
-        ap = {ap.x + fc * kp.x, ap.y + fc * kp.y};
-        if (rk.stage == rk.nstages - 1) {
-            if (rk.mean_level && idx == 0) ae = {rk.mean0[m], 0.0};
-            u[idx] = ae;
-            u[fsd + idx] = ap;
-            yn[idx] = ae;
-            yn[fsd + idx] = ap;
-        } else {
-            acc[idx] = ae;
-            acc[fsd + idx] = ap;
-            const double fb = dt * rk.b_next;
-            yn[idx] = {ue.x + fb * ke.x, ue.y + fb * ke.y};
-            yn[fsd + idx] = {up.x + fb * kp.x, up.y + fb * kp.y};
-        }
+        rk_epilogue(rk, kt, m, idx, kx, ky, NX, S[L::at(r, kx)], S[L::at(ROWS + r, kx)], scale, R, dt, y, fsd, hsum);
     }
     if (rk.hpart && rk.stage == 0 && rk.mode != EPI_STORE_SPEC2) {
         // deterministic block reduction: wave shuffle, then fixed-order sum of waves
@@ -913,6 +924,138 @@ __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, lo
             for (int w = 0; w < NT / 64; ++w) s += red[w];
             rk.hpart[m * gridDim.x + blockIdx.x] = s;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wave-per-row x inverse passes (the FP32 form at Nx = 512, 1024): each 64-lane
+// wave owns one ky row, lane l the modes kx = l + 64 i (i = 0..E-1), which
+// wave_fft transforms in registers, so the loads and stores of a row are
+// coalesced 64-lane accesses and no workgroup barrier is needed.  A workgroup
+// is XW waves on consecutive rows; grid (ceil(Nk/XW), groups, members); a wave
+// past row Nk-1 returns at once.
+constexpr int XW = NT / 64;
+
+template <class T, int LOGNX>
+__global__ __launch_bounds__(NT) void k_xinv_w(const cx<double> *__restrict__ Y, long sY, cx<T> *__restrict__ I,
+                                               long sI, long fs, XinvOps ops, KTab kt, const cx<T> *__restrict__ twx)
+{
+    constexpr int NX = 1 << LOGNX, E = NX / 64;
+    using L = Lds<T, NX>;
+    using R = T;  // multiplier arithmetic in the precision of the form
+    __shared__ cx<T> lds[XW * L::PITCH];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ky = blockIdx.x * XW + w;
+    if (ky >= kt.nk) return;
+    cx<T> *row = lds + w * L::PITCH;
+    const int grp = blockIdx.y, m = blockIdx.z;
+    const long r0 = (long)ky * NX;
+    const cx<double> *src = Y + m * sY + ops.src[grp] * fs + r0;
+    const double *tht = kt.th + m * kt.ths + r0;
+    const double h = kt.h(m), kyv = ky * kt.ky0;
+    cx<R> sr[E];
+    R kxr[E], kz[E], th[E];
+    {
+        cx<double> sd[E];
+        double kxv[E], thd[E];
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int kx = lane + 64 * i;
+            sd[i] = src[kx];
+            kxv[i] = kt.kx[kx];
+            if constexpr (sizeof(R) == 8) thd[i] = tht[kx];
+            else thd[i] = 1.0;
+        }
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            sr[i] = {R(sd[i].x), R(sd[i].y)};
+            kxr[i] = R(kxv[i]);
+            kz_th<R>(kxv[i], kyv, h, thd[i], kz[i], th[i]);  // d^n/dz^n: k^n tanh(kh)^(n odd)
+        }
+    }
+    const R kyr = R(kyv);
+    const int nops = ops.n[grp];
+    for (int o = 0; o < nops; ++o) {
+        const XinvOp op = ops.o[grp][o];
+        cx<T> v[E];
+        // the operator is uniform: one branch per output, not a select per mode
+        if (op.op == OP_DX) {
+#pragma unroll
+            for (int i = 0; i < E; ++i) v[i] = {T(-kxr[i] * sr[i].y), T(kxr[i] * sr[i].x)};
+        } else if (op.op == OP_DY) {
+#pragma unroll
+            for (int i = 0; i < E; ++i) v[i] = {T(-kyr * sr[i].y), T(kyr * sr[i].x)};
+        } else if (op.op == OP_DZ) {
+            R mm[E];
+#pragma unroll
+            for (int i = 0; i < E; ++i) mm[i] = (op.pow & 1) ? th[i] : R(1);
+            for (int p = 0; p < op.pow; ++p)
+#pragma unroll
+                for (int i = 0; i < E; ++i) mm[i] *= kz[i];
+#pragma unroll
+            for (int i = 0; i < E; ++i) v[i] = {T(mm[i] * sr[i].x), T(mm[i] * sr[i].y)};
+        } else {
+#pragma unroll
+            for (int i = 0; i < E; ++i) v[i] = {T(sr[i].x), T(sr[i].y)};
+        }
+        wave_fft<T, LOGNX, true>(v, row, twx, lane);
+        cx<T> *dst = I + m * sI + op.dst * fs + r0;
+#pragma unroll
+        for (int i = 0; i < E; ++i) dst[lane + 64 * i] = v[i];
+    }
+}
+
+// x forward pass of phi^(lev+1), d^j/dz^j for j = 1..ndst and the x inverse of
+// each, one row per wave: the spectral row stays in registers throughout
+template <class T, int LOGNX>
+__global__ __launch_bounds__(NT) void k_xfwdinv_w(cx<T> *__restrict__ I, long sI, long fs, int src, int dst0,
+                                                  int ndst, double scale, KTab kt, const cx<T> *__restrict__ twx)
+{
+    constexpr int NX = 1 << LOGNX, E = NX / 64;
+    using L = Lds<T, NX>;
+    using R = T;
+    __shared__ cx<T> lds[XW * L::PITCH];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ky = blockIdx.x * XW + w;
+    if (ky >= kt.nk) return;
+    cx<T> *row = lds + w * L::PITCH;
+    const int m = blockIdx.z;
+    const long r0 = (long)ky * NX;
+    cx<T> *base = I + m * sI;
+    const cx<T> *in = base + src * fs + r0;
+    const double *tht = kt.th + m * kt.ths + r0;
+    const double h = kt.h(m), kyv = ky * kt.ky0;
+    cx<T> s[E];
+    R kz[E], th[E];
+    {
+        double kxv[E], thd[E];
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int kx = lane + 64 * i;
+            s[i] = in[kx];
+            kxv[i] = kt.kx[kx];
+            if constexpr (sizeof(R) == 8) thd[i] = tht[kx];
+            else thd[i] = 1.0;
+        }
+#pragma unroll
+        for (int i = 0; i < E; ++i) kz_th<R>(kxv[i], kyv, h, thd[i], kz[i], th[i]);
+    }
+    wave_fft<T, LOGNX, false>(s, row, twx, lane);
+    R kj[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) kj[i] = R(scale);
+    for (int j = 1; j <= ndst; ++j) {
+        cx<T> v[E];
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            kj[i] *= kz[i];  // scale * |k|^j
+            const T mul = T((j & 1) ? kj[i] * th[i] : kj[i]);  // odd j: tanh(|k| h)
+            v[i] = {mul * s[i].x, mul * s[i].y};
+        }
+        wave_fft<T, LOGNX, true>(v, row, twx, lane);
+        cx<T> *out = base + (dst0 + j - 1) * fs + r0;
+#pragma unroll
+        for (int i = 0; i < E; ++i) out[lane + 64 * i] = v[i];
     }
 }
 

@@ -78,6 +78,27 @@ def test_rhs_2048_rows_vs_oracle(hos, oracle_mod, Nx, Ny, prec):
     assert err < (TOL64 if prec == 0 else TOL32), err
 
 
+@pytest.mark.parametrize("Nx", [512, 1024])
+def test_xpass_modes_agree(hos, oracle_mod, Nx):
+    """The x inverse passes with rows per workgroup and with one row per wave
+    (FP32 form) give the same RHS, both within the FP32 tolerance."""
+    Ny, nlevs, depth = 32, 3, 0.8
+    Lx, Ly = 2 * np.pi * Nx / 32, 2 * np.pi
+    print(f"x-pass modes {Nx}x{Ny}", flush=True)
+    o = oracle_mod.Oracle(Nx, Ny, nlevs, Lx, Ly, depth=depth)
+    s = make(hos, Nx, Ny, nlevs, 1, Lx=Lx, Ly=Ly, depth=depth)
+    u = random_state(Nx, Ny, nlevs, amp=0.05, slope=3.0, seed=5)
+    ref = o.rhs_hos(u)
+    s.set_tile(1, hos.HOS_TILE_XPASS)
+    assert s.get_tile(hos.HOS_TILE_XPASS) == 1
+    a = s.rhs(u)
+    s.set_tile(2, hos.HOS_TILE_XPASS)
+    assert s.get_tile(hos.HOS_TILE_XPASS) == 2
+    b = s.rhs(u)
+    assert rel_l2(a, ref) < TOL32 and rel_l2(b, ref) < TOL32, (rel_l2(a, ref), rel_l2(b, ref))
+    assert rel_l2(a, b) < 1e-6
+
+
 @pytest.mark.parametrize("nlevs", [1, 2, 4])
 def test_rhs_fp32_vs_oracle(hos, oracle_mod, nlevs):
     Nx, Ny = 128, 64
