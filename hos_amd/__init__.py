@@ -19,6 +19,7 @@ _lib = None
 
 HOS_FP64 = 0
 HOS_FP32 = 1
+HOS_OK, HOS_E_INVALID, HOS_E_DEVICE, HOS_E_NOMEM, HOS_E_STATE = 0, -1, -2, -3, -4
 
 
 class HosError(RuntimeError):
@@ -269,6 +270,15 @@ class Server:
 
     def drain(self, timeout=600.0):
         _check(self.lib.hos_server_drain(self.h, timeout))
+
+    def drain_for(self, seconds: float) -> bool:
+        """hos_server_drain for at most `seconds`: True when nothing is queued
+        or running any more, False when the time ran out first."""
+        rc = self.lib.hos_server_drain(self.h, seconds)
+        if rc == HOS_E_STATE:
+            return False
+        _check(rc)
+        return True
 
     def stats(self) -> dict:
         v = np.zeros(12)

@@ -265,7 +265,7 @@ struct Eng {
         return HOS_OK;
     }
     template <class PRO>
-    int yr2c_pair(const PRO &pro, int da, int db, int nin)
+    int yr2c_pair(const PRO &pro, int da, int nin)
     {
         const int tx = c->tx[HOS_TILE_PAIR];
         LOGN_SWITCH(c->logny, {
@@ -274,7 +274,7 @@ struct Eng {
                 dim3 grid(c->p.Nx / TXP, 1, nm);
                 Prof pf(c, HOS_K_YR2C_PAIR, (double)nm * sizeof(T) * (nin * (double)c->fsP + 4.0 * c->fs));
                 hipLaunchKernelGGL((k_yr2c_pair<T, LG, TXP, PRO>), grid, dim3(NT), 0, c->stream, pro, I(), sI(),
-                                   c->fs, da, db, c->p.Nx, twy());
+                                   c->fs, da, c->p.Nx, twy());
             })
         });
         return HOS_OK;
@@ -302,6 +302,7 @@ struct Eng {
         });
         return HOS_OK;
     }
+    // ib < 0: the interleaved pair of yr2c_pair at field ia
     int xfwd_rhs(int ia, int ib, const cx<double> *Y, const RkArgs &rk)
     {
         const double scale = 1.0 / ((double)c->p.Nx * c->p.Ny);
@@ -318,8 +319,8 @@ struct Eng {
             constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
             dim3 grid((c->Nk + ROWS - 1) / ROWS, 1, nm);
             Prof pf(c, HOS_K_XFWD_RHS, (double)nm * c->fs * (2.0 * 2.0 * sizeof(T) + 16.0 * f64));
-            hipLaunchKernelGGL((k_xfwd_rhs<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, I(), sI(), c->fs, ia, ib, Y,
-                               sY(), scale, ktab(c), rk, twx());
+            hipLaunchKernelGGL((k_xfwd_rhs<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, I(), sI(), c->fs, ia,
+                               ib < 0 ? ia : ib, ib < 0 ? 1 : 0, Y, sY(), scale, ktab(c), rk, twx());
         });
         return HOS_OK;
     }
@@ -383,7 +384,7 @@ struct Eng {
         int rc = w_levels(Y);
         if (rc) return rc;
         rc = rhs_pair();
-        if (!rc) rc = xfwd_rhs(0, 1, Y, rk);
+        if (!rc) rc = xfwd_rhs(0, -1, Y, rk);
         return rc;
     }
 };

@@ -888,9 +888,9 @@ static int zvel_impl(hos_ctx *c)
     rk.hpart = nullptr;
     for (int which = 0; which < 2 && !rc; ++which) {
         ProZvel<T> pro{e.phys(), c->ix, which, e.comp()};
-        rc = e.yr2c_pair(pro, 0, 1, c->nP - 4);
+        rc = e.yr2c_pair(pro, 0, c->nP - 4);
         rk.K = c->K + (long)which * 2 * c->fs;
-        if (!rc) rc = e.xfwd_rhs(0, 1, c->Y, rk);
+        if (!rc) rc = e.xfwd_rhs(0, -1, c->Y, rk);
     }
     return rc;
 }

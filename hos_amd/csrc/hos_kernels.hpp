@@ -630,11 +630,14 @@ __device__ __forceinline__ void r2c_tile(const PRO &pro, int m, cx<T> *__restric
 }
 
 // r2c along y of x-rows x0 .. x0+NROW-1 of two fields the prologue computes
-// together (the real and imaginary part of one complex FFT per x-row).
+// together (the real and imaginary part of one complex FFT per x-row).  The
+// two half spectra are written interleaved, o[(ky*nx + x)*2 + {0,1}], so each
+// (ky, x) is one 16-byte piece and a tile's piece of a ky line is contiguous
+// (separate fields would leave 8-byte pieces per tile, which cost whole
+// write sectors).
 template <class T, int LOGNY, int NROW, class PRO>
-__device__ __forceinline__ void r2c_pair_tile(const PRO &pro, int m, cx<T> *__restrict__ oa,
-                                              cx<T> *__restrict__ ob, int x0, int nx, cx<T> *buf,
-                                              const cx<T> *__restrict__ twy)
+__device__ __forceinline__ void r2c_pair_tile(const PRO &pro, int m, cx<T> *__restrict__ o, int x0, int nx,
+                                              cx<T> *buf, const cx<T> *__restrict__ twy)
 {
     using F = vec_t<T>;
     constexpr int V = Vec<T>::N;
@@ -659,8 +662,8 @@ __device__ __forceinline__ void r2c_pair_tile(const PRO &pro, int m, cx<T> *__re
         if (ky >= NK) continue;
         const cx<T> zk = buf[Lds<T, NY>::at(p, ky)];
         const cx<T> zm = buf[Lds<T, NY>::at(p, (NY - ky) & (NY - 1))];
-        oa[(long)ky * nx + x0 + p] = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
-        ob[(long)ky * nx + x0 + p] = {T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x)};
+        store2(o + 2 * ((long)ky * nx + x0 + p), cx<T>{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)},
+               cx<T>{T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x)});
     }
 }
 
@@ -676,16 +679,17 @@ __global__ __launch_bounds__(NT) void k_yr2c(PRO pro, cx<T> *__restrict__ I, lon
     r2c_tile<T, LOGNY, NPR, PRO>(pro, m, I + m * sI + dst * fsI, x0, nx, buf, twy);
 }
 
-// y r2c pass, two output fields at the same x.  grid (Nx/TXP, 1, members)
+// y r2c pass, two output fields at the same x, written interleaved into the
+// space of fields dst, dst+1.  grid (Nx/TXP, 1, members)
 template <class T, int LOGNY, int TXP, class PRO>
-__global__ __launch_bounds__(NT) void k_yr2c_pair(PRO pro, cx<T> *__restrict__ I, long sI, long fsI, int dstA,
-                                                  int dstB, int nx, const cx<T> *__restrict__ twy)
+__global__ __launch_bounds__(NT) void k_yr2c_pair(PRO pro, cx<T> *__restrict__ I, long sI, long fsI, int dst, int nx,
+                                                  const cx<T> *__restrict__ twy)
 {
     constexpr int NY = 1 << LOGNY;
     __shared__ cx<T> buf[Lds<T, NY>::size(TXP)];
     const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TXP;
     const int m = blockIdx.z;
-    r2c_pair_tile<T, LOGNY, TXP, PRO>(pro, m, I + m * sI + dstA * fsI, I + m * sI + dstB * fsI, x0, nx, buf, twy);
+    r2c_pair_tile<T, LOGNY, TXP, PRO>(pro, m, I + m * sI + dst * fsI, x0, nx, buf, twy);
 }
 
 // ---------------------------------------------------------------------------
@@ -765,7 +769,8 @@ __global__ __launch_bounds__(NT) void k_xfwdinv(cx<T> *__restrict__ I, long sI, 
 
 // ---------------------------------------------------------------------------
 // final x forward pass: N_eta, N_phi -> K = RHS (linear terms added in FP64,
-// ramp, dealias) -> RK stage update.
+// ramp, dealias) -> RK stage update.  The two inputs are fields iA and iB, or
+// (ilv) the interleaved pair that k_yr2c_pair wrote at field iA.
 enum : int { EPI_RK_SUBDIAG = 0, EPI_STORE_K = 1, EPI_STORE_SPEC2 = 2 };
 struct RkArgs {
     int mode;
@@ -865,7 +870,7 @@ __device__ __forceinline__ void rk_epilogue(const RkArgs &rk, const KTab &kt, in
 
 template <class T, int LOGNX, int ROWS>
 __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, long sI, long fs, int iA, int iB,
-                                                 const cx<double> *Y, long sY, double scale, KTab kt,
+                                                 int ilv, const cx<double> *Y, long sY, double scale, KTab kt,
                                                  RkArgs rk, const cx<T> *__restrict__ twx)
 {
     using G = XGeom<T, LOGNX>;
@@ -884,8 +889,12 @@ __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, lo
         for (int q = 0; q < PER; ++q) {
             const int e = threadIdx.x + q * NT;
             const long idx = (long)min(ky0 + e / NX, kt.nk - 1) * NX + e % NX;
-            va[q] = ia[idx];
-            vb[q] = ib[idx];
+            if (ilv) {
+                load2(ia + 2 * idx, va[q], vb[q]);  // the interleaved pair of k_yr2c_pair
+            } else {
+                va[q] = ia[idx];
+                vb[q] = ib[idx];
+            }
         }
 #pragma unroll
         for (int q = 0; q < PER; ++q) {

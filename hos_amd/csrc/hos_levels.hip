@@ -38,7 +38,7 @@ template <class T, int MO>
 static int rhs_pair_t(Eng<T> &e)
 {
     ProRhs<T, MO> pro{e.phys(), e.c->ix, e.comp()};
-    return e.yr2c_pair(pro, 0, 1, e.c->nP);
+    return e.yr2c_pair(pro, 0, e.c->nP);
 }
 
 template <class T>
