@@ -46,7 +46,7 @@ EXPORTS = [
     "hos_set_active", "hos_get_active", "hos_set_member", "hos_copy_member", "hos_step_async", "hos_wait",
     "hos_autotune", "hos_copy_bandwidth", "hos_get_tile",
 ]
-HOS_TILE_C2R, HOS_TILE_R2C, HOS_TILE_PAIR, HOS_TILE_ALL, HOS_TILE_XPASS = 0, 1, 2, 3, 4
+HOS_TILE_C2R, HOS_TILE_R2C, HOS_TILE_PAIR, HOS_TILE_ALL, HOS_TILE_XPASS, HOS_TILE_GROUPS = 0, 1, 2, 3, 4, 5
 HOS_OP_DX, HOS_OP_DY, HOS_OP_DZ, HOS_OP_SUM, HOS_OP_DEALIAS = 1, 2, 3, 4, 5
 # include/hos_ic.h
 IC_EXPORTS = ["hos_ic_jonswap", "hos_ic_uniform", "hos_ic_peak_period", "hos_dealias_state"]
@@ -441,7 +441,8 @@ class Solver:
     def set_tile(self, tx: int, kind: int = 3):
         """y-pass tile (4, 8, 16; 0 = default) for kind HOS_TILE_C2R (0),
         HOS_TILE_R2C (1), HOS_TILE_PAIR (2) or all (3); for HOS_TILE_XPASS (4)
-        the x-pass mode (1 rows per workgroup, 2 row per wave, 0 default)."""
+        the x-pass mode (1 rows per workgroup, 2 row per wave, 0 default); for
+        HOS_TILE_GROUPS (5) the member groups (1 or 2, 0 default)."""
         _check(self.lib.hos_set_tile(self.h, kind, tx))
 
     def get_tile(self, kind: int) -> int:

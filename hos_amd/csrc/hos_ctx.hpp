@@ -30,6 +30,12 @@ struct hos_ctx {
     int nI = 0, nP = 0;
     int tx[3] = {8, 4, 4};  // y tiles: c2r, r2c, pair
     int xwave = 0;          // x inverse passes one row per wave (where available)
+    // member groups: with 2, members [0, n0) and [n0, active) run as two
+    // independent launch chains on their own streams (gst), forked from and
+    // joined to the context stream each step
+    int ngroups = 1;
+    hipStream_t gst[2] = {nullptr, nullptr};
+    hipEvent_t gfork = nullptr, gjoin[2] = {nullptr, nullptr};
     bool f32 = false;
     long fs = 0;    // complex elements per spectral field  (Nk*Nx)
     long fsP = 0;   // real elements per physical field     (Nx*Ny)
@@ -42,7 +48,7 @@ struct hos_ctx {
            *mean0 = nullptr, *depth = nullptr;
     double *th = nullptr;  // [nm][Nk][Nx] tanh(|k| h) per member
     int active = 0;  // members 0..active-1 are stepped
-    std::vector<hipGraphExec_t> gexec_by_active;
+    std::vector<hipGraphExec_t> gexec_by_active;  // [graph_slot(active, groups, group)]
     std::vector<hipGraph_t> graph_by_active;
     bool async_pending = false;
     int async_steps = 0;
@@ -86,25 +92,26 @@ inline hipEvent_t pool_event(hos_ctx *c)
     return e;
 }
 
-// Records a HIP event pair around one kernel launch on the context's stream
+// Records a HIP event pair around one kernel launch on the launch's stream
 // when kernel profiling is on (eager launches only).
 struct Prof {
     hos_ctx *c;
+    hipStream_t st;
     int kind;
     double bytes;
     hipEvent_t a = nullptr;
-    Prof(hos_ctx *c_, int k, double by) : c(c_), kind(k), bytes(by)
+    Prof(hos_ctx *c_, hipStream_t s_, int k, double by) : c(c_), st(s_), kind(k), bytes(by)
     {
         if (c->prof_on && !c->capturing) {
             a = pool_event(c);
-            (void)hipEventRecord(a, c->stream);
+            (void)hipEventRecord(a, st);
         }
     }
     ~Prof()
     {
         if (a) {
             hipEvent_t b = pool_event(c);
-            (void)hipEventRecord(b, c->stream);
+            (void)hipEventRecord(b, st);
             c->recs.push_back({kind, a, b, bytes});
         }
     }
@@ -179,9 +186,20 @@ inline int x_nparts(const hos_ctx *c)
 template <class T>
 struct Eng {
     hos_ctx *c;
-    int nm;  // members in this launch (from member 0)
-    cx<T> *I() const { return (cx<T> *)c->I; }
-    T *P() const { return (T *)c->P; }
+    int nm;                  // members in this launch
+    int m0 = 0;              // first member (the launch covers m0 .. m0+nm-1)
+    hipStream_t st = nullptr;  // launch stream (null: the context's)
+    hipStream_t S() const { return st ? st : c->stream; }
+    cx<T> *I() const { return (cx<T> *)c->I + (long)m0 * sI(); }
+    T *P() const { return (T *)c->P + (long)m0 * sP(); }
+    // wavenumber tables of members m0.. (the kernels index them by blockIdx.z)
+    KTab kt() const
+    {
+        KTab k = ktab(c);
+        k.depth += m0;
+        k.th += (long)m0 * k.ths;
+        return k;
+    }
     long sI() const { return (long)c->nI * c->fs; }
     long sP() const { return (long)c->nP * c->fsP; }
     long sY() const { return 2 * c->fs; }
@@ -220,19 +238,19 @@ struct Eng {
             dim3 grid((c->Nk + ROWS - 1) / ROWS, ngroups, nm);
             // FP64 form: the tanh(|k| h) table is read once per group
             const double thb = sizeof(T) == 8 ? 8.0 * ngroups : 0.0;
-            Prof pf(c, HOS_K_XINV, (double)nm * c->fs * (16.0 * ngroups + thb + 2.0 * sizeof(T) * nops));
+            Prof pf(c, S(), HOS_K_XINV, (double)nm * c->fs * (16.0 * ngroups + thb + 2.0 * sizeof(T) * nops));
             bool waved = false;
             if constexpr (x_wave<T>(LG)) {
                 if (c->xwave) {
                     dim3 gw((c->Nk + XW - 1) / XW, ngroups, nm);
-                    hipLaunchKernelGGL((k_xinv_w<T, LG>), gw, dim3(NT), 0, c->stream, Y, sY(), I(), sI(), c->fs, ops,
-                                       ktab(c), twx());
+                    hipLaunchKernelGGL((k_xinv_w<T, LG>), gw, dim3(NT), 0, S(), Y, sY(), I(), sI(), c->fs, ops,
+                                       kt(), twx());
                     waved = true;
                 }
             }
             if (!waved) {
-                hipLaunchKernelGGL((k_xinv<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, Y, sY(), I(), sI(), c->fs,
-                                   ops, ktab(c), twx());
+                hipLaunchKernelGGL((k_xinv<T, LG, ROWS>), grid, dim3(NT), 0, S(), Y, sY(), I(), sI(), c->fs,
+                                   ops, kt(), twx());
             }
         });
         return HOS_OK;
@@ -243,8 +261,8 @@ struct Eng {
         LOGN_SWITCH(c->logny, {
             TX_SWITCH(T, tx, {
                 dim3 grid(c->p.Nx / TXV, nf, nm);
-                Prof pf(c, HOS_K_YC2R, (double)nm * nf * sizeof(T) * (2.0 * c->fs + c->fsP));
-                hipLaunchKernelGGL((k_yc2r<T, LG, TXV>), grid, dim3(NT), 0, c->stream, I(), sI(), c->fs, P(), sP(),
+                Prof pf(c, S(), HOS_K_YC2R, (double)nm * nf * sizeof(T) * (2.0 * c->fs + c->fsP));
+                hipLaunchKernelGGL((k_yc2r<T, LG, TXV>), grid, dim3(NT), 0, S(), I(), sI(), c->fs, P(), sP(),
                                    c->fsP, fm, c->p.Nx, twy());
             })
         });
@@ -257,8 +275,8 @@ struct Eng {
         LOGN_SWITCH(c->logny, {
             TX_SWITCH(T, tx, {
                 dim3 grid(c->p.Nx / TXV, 1, nm);
-                Prof pf(c, HOS_K_YR2C, (double)nm * sizeof(T) * (nin * (double)c->fsP + 2.0 * c->fs));
-                hipLaunchKernelGGL((k_yr2c<T, LG, TXV, PRO>), grid, dim3(NT), 0, c->stream, pro, I(), sI(), c->fs,
+                Prof pf(c, S(), HOS_K_YR2C, (double)nm * sizeof(T) * (nin * (double)c->fsP + 2.0 * c->fs));
+                hipLaunchKernelGGL((k_yr2c<T, LG, TXV, PRO>), grid, dim3(NT), 0, S(), pro, I(), sI(), c->fs,
                                    dst, c->p.Nx, twy());
             })
         });
@@ -272,8 +290,8 @@ struct Eng {
             TX_SWITCH(T, tx, {
                 constexpr int TXP = TXV / 2;
                 dim3 grid(c->p.Nx / TXP, 1, nm);
-                Prof pf(c, HOS_K_YR2C_PAIR, (double)nm * sizeof(T) * (nin * (double)c->fsP + 4.0 * c->fs));
-                hipLaunchKernelGGL((k_yr2c_pair<T, LG, TXP, PRO>), grid, dim3(NT), 0, c->stream, pro, I(), sI(),
+                Prof pf(c, S(), HOS_K_YR2C_PAIR, (double)nm * sizeof(T) * (nin * (double)c->fsP + 4.0 * c->fs));
+                hipLaunchKernelGGL((k_yr2c_pair<T, LG, TXP, PRO>), grid, dim3(NT), 0, S(), pro, I(), sI(),
                                    c->fs, da, c->p.Nx, twy());
             })
         });
@@ -285,19 +303,19 @@ struct Eng {
         LOGN_SWITCH(c->lognx, {
             constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
             dim3 grid((c->Nk + ROWS - 1) / ROWS, 1, nm);
-            Prof pf(c, HOS_K_XFWDINV, (double)nm * c->fs * (2.0 * sizeof(T) * (1 + ndst) + (sizeof(T) == 8 ? 8.0 : 0.0)));
+            Prof pf(c, S(), HOS_K_XFWDINV, (double)nm * c->fs * (2.0 * sizeof(T) * (1 + ndst) + (sizeof(T) == 8 ? 8.0 : 0.0)));
             bool waved = false;
             if constexpr (x_wave<T>(LG)) {
                 if (c->xwave) {
                     dim3 gw((c->Nk + XW - 1) / XW, 1, nm);
-                    hipLaunchKernelGGL((k_xfwdinv_w<T, LG>), gw, dim3(NT), 0, c->stream, I(), sI(), c->fs, src, dst0,
-                                       ndst, scale, ktab(c), twx());
+                    hipLaunchKernelGGL((k_xfwdinv_w<T, LG>), gw, dim3(NT), 0, S(), I(), sI(), c->fs, src, dst0,
+                                       ndst, scale, kt(), twx());
                     waved = true;
                 }
             }
             if (!waved) {
-                hipLaunchKernelGGL((k_xfwdinv<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, I(), sI(), c->fs, src, dst0,
-                                   ndst, scale, ktab(c), twx());
+                hipLaunchKernelGGL((k_xfwdinv<T, LG, ROWS>), grid, dim3(NT), 0, S(), I(), sI(), c->fs, src, dst0,
+                                   ndst, scale, kt(), twx());
             }
         });
         return HOS_OK;
@@ -318,9 +336,9 @@ struct Eng {
         LOGN_SWITCH(c->lognx, {
             constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
             dim3 grid((c->Nk + ROWS - 1) / ROWS, 1, nm);
-            Prof pf(c, HOS_K_XFWD_RHS, (double)nm * c->fs * (2.0 * 2.0 * sizeof(T) + 16.0 * f64));
-            hipLaunchKernelGGL((k_xfwd_rhs<T, LG, ROWS>), grid, dim3(NT), 0, c->stream, I(), sI(), c->fs, ia,
-                               ib < 0 ? ia : ib, ib < 0 ? 1 : 0, Y, sY(), scale, ktab(c), rk, twx());
+            Prof pf(c, S(), HOS_K_XFWD_RHS, (double)nm * c->fs * (2.0 * 2.0 * sizeof(T) + 16.0 * f64));
+            hipLaunchKernelGGL((k_xfwd_rhs<T, LG, ROWS>), grid, dim3(NT), 0, S(), I(), sI(), c->fs, ia,
+                               ib < 0 ? ia : ib, ib < 0 ? 1 : 0, Y, sY(), scale, kt(), rk, twx());
         });
         return HOS_OK;
     }

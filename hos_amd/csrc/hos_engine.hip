@@ -150,6 +150,11 @@ static int default_tx(bool f32, int logny, int kind)
 // member runs faster with rows per workgroup)
 static int default_xwave(const hos_ctx *c) { return x_wave_ok(c) && c->nm >= 4 ? 1 : 0; }
 
+// two member groups on their own streams by default from 4 members (measured
+// at 8 x 1024^2: 6% faster than one chain; kernel tails of one chain overlap
+// the other's work)
+static int default_groups(const hos_ctx *c) { return c->nm >= 4 ? 2 : 1; }
+
 static int validate(const hos_params *p, int nm)
 {
     if (!p) return fail(HOS_E_INVALID, "null params");
@@ -252,6 +257,11 @@ static void release(hos_ctx *c)
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    for (int g = 0; g < 2; ++g) {
+        if (c->gst[g]) (void)hipStreamDestroy(c->gst[g]);
+        if (c->gjoin[g]) (void)hipEventDestroy(c->gjoin[g]);
+    }
+    if (c->gfork) (void)hipEventDestroy(c->gfork);
 }
 
 
@@ -272,6 +282,7 @@ extern "C" int hos_create(const hos_params *p, int nmembers, hos_ctx **out)
     c->nK = c->subdiag ? 2 : (c->S > 2 ? c->S : 2);
     for (int k = HOS_TILE_C2R; k <= HOS_TILE_PAIR; ++k) c->tx[k] = default_tx(c->f32, c->logny, k);
     c->xwave = default_xwave(c);
+    c->ngroups = default_groups(c);
     const int nm = nmembers;
     auto bail = [&](int code) {
         release(c);
@@ -280,6 +291,13 @@ extern "C" int hos_create(const hos_params *p, int nmembers, hos_ctx **out)
     };
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(HOS_E_DEVICE, "hipStreamCreate failed"));
+    for (int g = 0; g < 2; ++g) {
+        if (hipStreamCreateWithFlags(&c->gst[g], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c->gjoin[g], hipEventDisableTiming) != hipSuccess)
+            return bail(fail(HOS_E_DEVICE, "group stream set-up failed"));
+    }
+    if (hipEventCreateWithFlags(&c->gfork, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(HOS_E_DEVICE, "group event set-up failed"));
     (void)hipEventCreate(&c->ev0);
     (void)hipEventCreate(&c->ev1);
     const size_t rb = c->f32 ? 4 : 8;
@@ -337,8 +355,8 @@ extern "C" int hos_create(const hos_params *p, int nmembers, hos_ctx **out)
         for (int m = 0; m < nm; ++m)
             if ((rc = th_table(c, m))) return bail(rc);
     }
-    c->gexec_by_active.assign(nm + 1, nullptr);
-    c->graph_by_active.assign(nm + 1, nullptr);
+    c->gexec_by_active.assign(3 * (nm + 1), nullptr);
+    c->graph_by_active.assign(3 * (nm + 1), nullptr);
     if (hipDeviceSynchronize() != hipSuccess) return bail(fail(HOS_E_DEVICE, "device sync failed"));
     *out = c;
     return HOS_OK;
@@ -378,47 +396,95 @@ static RkArgs rk_base(hos_ctx *c)
     return rk;
 }
 
-template <class T>
-static int launch_step(hos_ctx *c)
+// groups this step runs in (1 or 2); the generic Butcher path and the
+// per-kernel profile run as one chain
+static int groups_now(const hos_ctx *c)
 {
-    Eng<T> e{c, c->active};
+    return (c->ngroups == 2 && c->subdiag && c->active >= 4 && !c->prof_on) ? 2 : 1;
+}
+// graph slot of (active count, group count, group)
+static size_t graph_slot(const hos_ctx *c, int ng, int g) { return 3 * (size_t)c->active + (ng == 1 ? 0 : 1 + g); }
+
+static void group_range(const hos_ctx *c, int g, int ng, int *m0, int *n)
+{
+    const int n0 = ng == 1 ? c->active : (c->active + 1) / 2;
+    *m0 = g == 0 ? 0 : n0;
+    *n = g == 0 ? n0 : c->active - n0;
+}
+
+// one RK step of members m0 .. m0+n-1, launched on stream st
+template <class T>
+static int launch_step(hos_ctx *c, int m0, int n, hipStream_t st)
+{
+    Eng<T> e{c, n, m0, st};
     const int nparts = x_nparts(c);
+    const long s2 = 2 * c->fs;  // member stride of the FP64 [2][Nk][Nx] arrays
     int rc = HOS_OK;
     for (int s = 0; s < c->S && !rc; ++s) {
         RkArgs rk = rk_base(c);
         rk.stage = s;
         rk.c_s = c->c[s];
+        rk.t += m0;
+        rk.dt += m0;
+        rk.U += m0 * s2;
+        rk.A += m0 * s2;
+        rk.Yn += m0 * s2;
+        rk.hpart += (long)m0 * nparts;
+        rk.vol += m0;
+        rk.mean0 += m0;
         if (c->subdiag) {
             rk.mode = EPI_RK_SUBDIAG;
             rk.b_next = (s + 1 < c->S) ? c->b[s + 1][s] : 0.0;
-            rc = e.rhs(c->Y, rk);
+            rc = e.rhs(c->Y + m0 * s2, rk);
         } else {
-            // generic Butcher: K_s stored, then Y_{s+1} = u + dt sum_p b[s+1][p] K_p
+            // generic Butcher (one chain, m0 = 0): K_s stored, then
+            // Y_{s+1} = u + dt sum_p b[s+1][p] K_p
             rk.mode = EPI_STORE_K;
-            rk.K = c->K + (long)s * c->nm * 2 * c->fs;
+            rk.K = c->K + (long)s * c->nm * s2;
             rc = e.rhs(c->Y, rk);
             CombArgs ca{};
             const bool last = (s == c->S - 1);
             ca.np = s + 1;
             for (int q = 0; q <= s; ++q) ca.coef[q] = last ? c->c[q] : c->b[s + 1][q];
-            const long n = 2 * c->fs;
-            dim3 grid((unsigned)std::min<long>((n + NT - 1) / NT, 2048), c->active);
+            dim3 grid((unsigned)std::min<long>((s2 + NT - 1) / NT, 2048), n);
             if (!rc) {
-                hipLaunchKernelGGL(k_combine, grid, dim3(NT), 0, c->stream, c->U, c->K, (long)c->nm * n, n,
-                                   last ? c->U : c->Y, n, c->dt, ca);
+                hipLaunchKernelGGL(k_combine, grid, dim3(NT), 0, e.S(), c->U, c->K, (long)c->nm * s2, s2,
+                                   last ? c->U : c->Y, s2, c->dt, ca);
                 if (last)
-                    hipLaunchKernelGGL(k_copy_c64, dim3(2048), dim3(NT), 0, c->stream, c->U, c->Y,
-                                       (long)c->active * n);
+                    hipLaunchKernelGGL(k_copy_c64, dim3(2048), dim3(NT), 0, e.S(), c->U, c->Y, (long)n * s2);
             }
         }
     }
     if (!rc)
-        hipLaunchKernelGGL(k_end_step, dim3(c->active), dim3(NT), 0, c->stream, c->hpart, nparts, c->H, c->t,
-                           c->dt, 1);
+        hipLaunchKernelGGL(k_end_step, dim3(n), dim3(NT), 0, e.S(), c->hpart + (long)m0 * nparts, nparts,
+                           c->H + m0, c->t + m0, c->dt + m0, 1);
     return rc;
 }
 
-static int step_once(hos_ctx *c) { return c->f32 ? launch_step<float>(c) : launch_step<double>(c); }
+static int launch_group(hos_ctx *c, int g, int ng, hipStream_t st)
+{
+    int m0 = 0, n = 0;
+    group_range(c, g, ng, &m0, &n);
+    return c->f32 ? launch_step<float>(c, m0, n, st) : launch_step<double>(c, m0, n, st);
+}
+
+// one step, stream-ordered: one chain on the context stream, or two groups
+// forked from it and joined back
+static int step_once(hos_ctx *c)
+{
+    const int ng = groups_now(c);
+    if (ng == 1) return launch_group(c, 0, 1, c->stream);
+    HIP_TRY(hipEventRecord(c->gfork, c->stream));
+    for (int g = 0; g < ng; ++g) {
+        HIP_TRY(hipStreamWaitEvent(c->gst[g], c->gfork, 0));
+        int rc = launch_group(c, g, ng, c->gst[g]);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(c->gjoin[g], c->gst[g]));
+        HIP_TRY(hipStreamWaitEvent(c->stream, c->gjoin[g], 0));
+    }
+    return HOS_OK;
+}
+
 
 extern "C" int hos_use_graph(hos_ctx *c, int enable)
 {
@@ -437,23 +503,49 @@ static void drop_graphs(hos_ctx *c)
     }
 }
 
-// one captured step per active-member count (grid sizes are baked in)
+// one captured step per (active-member count, group): grid sizes are baked in.
+// Each group's chain is its own graph, launched on the group's stream, so the
+// two chains run concurrently.
 static int build_graph(hos_ctx *c)
 {
-    if (c->gexec_by_active[c->active]) return HOS_OK;
-    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    c->capturing = true;
-    int rc = step_once(c);
-    c->capturing = false;
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(c->stream, &g);
-    if (rc) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
+    const int ng = groups_now(c);
+    for (int g = 0; g < ng; ++g) {
+        const size_t slot = graph_slot(c, ng, g);
+        if (c->gexec_by_active[slot]) continue;
+        hipStream_t st = ng == 1 ? c->stream : c->gst[g];
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        c->capturing = true;
+        int rc = launch_group(c, g, ng, st);
+        c->capturing = false;
+        hipGraph_t gr = nullptr;
+        hipError_t e = hipStreamEndCapture(st, &gr);
+        if (rc) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(HOS_E_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+        c->graph_by_active[slot] = gr;
+        HIP_TRY(hipGraphInstantiate(&c->gexec_by_active[slot], gr, nullptr, nullptr, 0));
     }
-    if (e != hipSuccess) return fail(HOS_E_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
-    c->graph_by_active[c->active] = g;
-    HIP_TRY(hipGraphInstantiate(&c->gexec_by_active[c->active], g, nullptr, nullptr, 0));
+    return HOS_OK;
+}
+
+// one step by graph replay: one graph on the context stream, or the groups'
+// graphs forked from it and joined back
+static int replay_step(hos_ctx *c)
+{
+    const int ng = groups_now(c);
+    if (ng == 1) {
+        HIP_TRY(hipGraphLaunch(c->gexec_by_active[graph_slot(c, 1, 0)], c->stream));
+        return HOS_OK;
+    }
+    HIP_TRY(hipEventRecord(c->gfork, c->stream));
+    for (int g = 0; g < ng; ++g) {
+        HIP_TRY(hipStreamWaitEvent(c->gst[g], c->gfork, 0));
+        HIP_TRY(hipGraphLaunch(c->gexec_by_active[graph_slot(c, ng, g)], c->gst[g]));
+        HIP_TRY(hipEventRecord(c->gjoin[g], c->gst[g]));
+        HIP_TRY(hipStreamWaitEvent(c->stream, c->gjoin[g], 0));
+    }
     return HOS_OK;
 }
 
@@ -475,7 +567,8 @@ static int step_enqueue(hos_ctx *c, int nsteps)
     HIP_TRY(hipEventRecord(c->step_ev[0], c->stream));
     for (int n = 0; n < nsteps && c->active > 0; ++n) {
         if (c->use_graph) {
-            HIP_TRY(hipGraphLaunch(c->gexec_by_active[c->active], c->stream));
+            int rc = replay_step(c);
+            if (rc) return rc;
         } else {
             int rc = step_once(c);
             if (rc) return rc;
@@ -645,6 +738,13 @@ extern "C" int hos_sync(hos_ctx *c)
 extern "C" int hos_set_tile(hos_ctx *c, int kind, int tx)
 {
     if (!c) return fail(HOS_E_INVALID, "null context");
+    if (kind == HOS_TILE_GROUPS) {
+        if (tx < 0 || tx > 2) return fail(HOS_E_INVALID, "member groups %d: 0 default, 1 or 2", tx);
+        if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
+        c->ngroups = tx == 0 ? default_groups(c) : tx;
+        drop_graphs(c);
+        return HOS_OK;
+    }
     if (kind == HOS_TILE_XPASS) {
         if (tx < 0 || tx > 2) return fail(HOS_E_INVALID, "x-pass mode %d: 0 default, 1 rows per workgroup, 2 row per wave", tx);
         if (tx == 2 && !x_wave_ok(c)) return fail(HOS_E_INVALID, "row-per-wave x passes need the FP32 form and Nx of 512 or 1024");
@@ -672,6 +772,7 @@ extern "C" int hos_get_tile(hos_ctx *c, int kind)
 {
     if (!c) return fail(HOS_E_INVALID, "null context");
     if (kind == HOS_TILE_XPASS) return c->xwave ? 2 : 1;
+    if (kind == HOS_TILE_GROUPS) return c->ngroups;
     if (kind < HOS_TILE_C2R || kind > HOS_TILE_PAIR) return fail(HOS_E_INVALID, "unknown tile kind %d", kind);
     return c->tx[kind];
 }
@@ -680,7 +781,7 @@ extern "C" int hos_get_tile(hos_ctx *c, int kind)
 // autotuner: y tiles per kernel kind, then the x-pass mode, timed on the
 // context's own members
 static std::mutex g_tune_mu;
-static std::map<std::tuple<int, int, int, int, int, int>, std::array<int, 4>> g_tune_cache;
+static std::map<std::tuple<int, int, int, int, int, int>, std::array<int, 5>> g_tune_cache;
 
 extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
 {
@@ -694,6 +795,7 @@ extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
         if (it != g_tune_cache.end()) {
             for (int k = 0; k < 3; ++k) c->tx[k] = it->second[k];
             c->xwave = it->second[3];
+            c->ngroups = it->second[4];
             drop_graphs(c);
             if (tiles_out)
                 for (int k = 0; k < 3; ++k) tiles_out[k] = c->tx[k];
@@ -715,7 +817,7 @@ extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
     for (int m = 0; m < c->nm; ++m) dtev[m] = c->host_dt[m] > 0 ? c->host_dt[m] : 1e-3;
     int rc = upload_dt(c, dtev.data());
     const int cand[3] = {4, 8, 16};
-    std::array<int, 4> best = {c->tx[0], c->tx[1], c->tx[2], c->xwave};
+    std::array<int, 5> best = {c->tx[0], c->tx[1], c->tx[2], c->xwave, c->ngroups};
     double best_ms = 1e30;
     const int saved_graph = c->use_graph;
     c->use_graph = 1;
@@ -741,7 +843,7 @@ extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
                 rc = trial(ms);
                 if (!rc && ms < best_ms) {
                     best_ms = ms;
-                    best = {tt[0], tt[1], tt[2], c->xwave};
+                    best = {tt[0], tt[1], tt[2], c->xwave, c->ngroups};
                 }
             }
     if (!rc && x_wave_ok(c)) {
@@ -755,9 +857,22 @@ extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
             best[3] = c->xwave;
         }
     }
+    if (!rc && c->subdiag && c->active >= 4) {
+        // the other member-group count with the best tiles and x-pass mode
+        for (int k = 0; k < 3; ++k) c->tx[k] = best[k];
+        c->xwave = best[3];
+        c->ngroups = best[4] == 2 ? 1 : 2;
+        double ms = 0.0;
+        rc = trial(ms);
+        if (!rc && ms < best_ms) {
+            best_ms = ms;
+            best[4] = c->ngroups;
+        }
+    }
     c->use_graph = saved_graph;
     for (int k = 0; k < 3; ++k) c->tx[k] = best[k];
     c->xwave = best[3];
+    c->ngroups = best[4];
     drop_graphs(c);
     (void)hipMemcpyAsync(c->U, saveU, st, hipMemcpyDeviceToDevice, c->stream);
     (void)hipMemcpyAsync(c->Y, saveY, st, hipMemcpyDeviceToDevice, c->stream);
