@@ -297,7 +297,8 @@ struct Eng {
         });
         return HOS_OK;
     }
-    int xfwdinv(int src, int dst0, int ndst)
+    // outputs j = 1..ndst to fields dst0+j-1, the last one to dtop if dtop >= 0
+    int xfwdinv(int src, int dst0, int ndst, int dtop = -1)
     {
         const double scale = 1.0 / ((double)c->p.Nx * c->p.Ny);
         LOGN_SWITCH(c->lognx, {
@@ -309,13 +310,13 @@ struct Eng {
                 if (c->xwave) {
                     dim3 gw((c->Nk + XW - 1) / XW, 1, nm);
                     hipLaunchKernelGGL((k_xfwdinv_w<T, LG>), gw, dim3(NT), 0, S(), I(), sI(), c->fs, src, dst0,
-                                       ndst, scale, kt(), twx());
+                                       ndst, dtop, scale, kt(), twx());
                     waved = true;
                 }
             }
             if (!waved) {
                 hipLaunchKernelGGL((k_xfwdinv<T, LG, ROWS>), grid, dim3(NT), 0, S(), I(), sI(), c->fs, src, dst0,
-                                   ndst, scale, kt(), twx());
+                                   ndst, dtop, scale, kt(), twx());
             }
         });
         return HOS_OK;
@@ -343,66 +344,112 @@ struct Eng {
         return HOS_OK;
     }
 
+    // x-intermediate slots of the fields the RHS pass transforms itself
+    // (deferred; see k_yr2c_rhs): after the working slots 0 .. max(2, M)-1
+    int dslot(int f) const { return std::max(2, c->M) + f; }
+    // slot of the W^(M) term k (d^{k+1} phi^(M-k)/dz^{k+1}) among them
+    int dslot_w(int k) const { return dslot(4 + k); }
+
     // level 0: eta, eta_x, eta_y, phi_x, phi_y, d^j phi/dz^j (j = 1..M), the
-    // outputs of the first x inverse pass; returns their count
-    int level0(std::vector<XinvOp> &ops, FieldMap &fm)
+    // outputs of the first x inverse pass; the ones bound for the physical
+    // grid are listed in fm, their count returned.  defer: eta_x, eta_y,
+    // phi_x, phi_y and d^M phi/dz^M stay x-intermediates for the RHS pass.
+    int level0(std::vector<XinvOp> &ops, FieldMap &fm, bool defer)
     {
         const HosIdx &ix = c->ix;
-        int n = 0;
+        const int M = c->M;
+        int n = 0, slot = 0;
         auto add = [&](int src, int op, int pw, int pdst) {
-            ops.push_back({src, op, pw, n});
-            fm.src[n] = n;
-            fm.dst[n] = pdst;
-            ++n;
+            ops.push_back({src, op, pw, slot});
+            fm.src[n] = slot++;
+            fm.dst[n++] = pdst;
         };
+        auto keep = [&](int src, int op, int pw, int dst) { ops.push_back({src, op, pw, dst}); };
         add(0, OP_ID, 0, ix.eta);
-        add(0, OP_DX, 0, ix.etax);
-        add(0, OP_DY, 0, ix.etay);
-        add(1, OP_DX, 0, ix.phix);
-        add(1, OP_DY, 0, ix.phiy);
-        for (int j = 1; j <= c->M; ++j) add(1, OP_DZ, j, ix.d[1][j]);
+        if (defer) {
+            keep(0, OP_DX, 0, dslot(0));
+            keep(0, OP_DY, 0, dslot(1));
+            keep(1, OP_DX, 0, dslot(2));
+            keep(1, OP_DY, 0, dslot(3));
+            for (int j = 1; j < M; ++j) add(1, OP_DZ, j, ix.d[1][j]);
+            keep(1, OP_DZ, M, dslot_w(M - 1));
+        } else {
+            add(0, OP_DX, 0, ix.etax);
+            add(0, OP_DY, 0, ix.etay);
+            add(1, OP_DX, 0, ix.phix);
+            add(1, OP_DY, 0, ix.phiy);
+            for (int j = 1; j <= M; ++j) add(1, OP_DZ, j, ix.d[1][j]);
+        }
         return n;
     }
 
     // y r2c of phi^(lev+1) (prologue instantiated per level) and the RHS
-    // pair pass (prologue instantiated per order); defined in hos_levels.hip
+    // pass (instantiated per order); defined in hos_levels.hip
     int phi_level(int lev);
     int rhs_pair();
 
-    // physical fields of the W expansion from the stage input Y
-    int w_levels(const cx<double> *Y)
+    // physical fields of the W expansion from the stage input Y; defer: the
+    // fields only the RHS pass reads stay x-intermediates (dslot)
+    int w_levels(const cx<double> *Y, bool defer = false)
     {
         const HosIdx &ix = c->ix;
         const int M = c->M;
         std::vector<XinvOp> ops;
         FieldMap fm{};
-        const int n = level0(ops, fm);
+        const int n = level0(ops, fm, defer);
         int rc = xinv(Y, ops);
         if (!rc) rc = yc2r(fm, n);
         for (int lev = 1; lev <= M - 1 && !rc; ++lev) {
             rc = phi_level(lev);
             const int nd = M - lev;
-            if (!rc) rc = xfwdinv(0, 1, nd);
+            // the top derivative d^{M-lev} phi^(lev+1) is the W^(M) term k = M-lev-1
+            if (!rc) rc = xfwdinv(0, 1, nd, defer ? dslot_w(M - lev - 1) : -1);
+            const int np = defer ? nd - 1 : nd;
             FieldMap f2{};
-            for (int j = 1; j <= nd; ++j) {
+            for (int j = 1; j <= np; ++j) {
                 f2.src[j - 1] = j;
                 f2.dst[j - 1] = ix.d[lev + 1][j];
             }
-            if (!rc) rc = yc2r(f2, nd);
+            if (!rc && np > 0) rc = yc2r(f2, np);
         }
         return rc;
     }
 
-    // RHS: the W levels (each field its own workgroups), the two RHS fields'
-    // y r2c, and the x forward pass with the RK epilogue.  (Fusing a level's
-    // y c2r with the next y r2c in one kernel serialises the fields of a
-    // level inside a workgroup and measured slower.)
+    // RHS: the W levels (each field its own workgroups; the fields only the
+    // RHS pass reads are left as x-intermediates), the RHS pass (their y c2r,
+    // N_eta and N_phi, y r2c), and the x forward pass with the RK epilogue.
+    // (Fusing a level's y c2r with the next y r2c in one kernel serialises
+    // the fields of a level inside a workgroup and measured slower.)
     int rhs(const cx<double> *Y, const RkArgs &rk)
     {
-        int rc = w_levels(Y);
+        int rc = w_levels(Y, true);
         if (rc) return rc;
         rc = rhs_pair();
         if (!rc) rc = xfwd_rhs(0, -1, Y, rk);
         return rc;
+    }
+
+    // the RHS pass of order MO (k_yr2c_rhs); TXP = half the pair tile, at most
+    // what keeps four 16-byte tasks per thread
+    template <int MO>
+    int yr2c_rhs()
+    {
+        RhsDeferred<MO> dfr;
+        for (int f = 0; f < MO + 4; ++f) dfr.slot[f] = dslot(f);
+        const int nin = 1 + (MO - 1) * MO / 2;  // eta and d^j phi^(a), a + j <= M, from the physical grid
+        const int tx = c->tx[HOS_TILE_PAIR];
+        LOGN_SWITCH(c->logny, {
+            TX_SWITCH(T, tx, {
+                constexpr int V = Vec<T>::N;
+                constexpr int TXP0 = TXV / 2;
+                constexpr int TXP = (TXP0 > 2 && TXP0 * (1 << LG) / V > 4 * NT) ? 2 : TXP0;
+                dim3 grid(c->p.Nx / TXP, 1, nm);
+                Prof pf(c, S(), HOS_K_YR2C_PAIR,
+                        (double)nm * sizeof(T) * (nin * (double)c->fsP + 2.0 * (MO + 4) * c->fs + 4.0 * c->fs));
+                hipLaunchKernelGGL((k_yr2c_rhs<T, LG, TXP, MO>), grid, dim3(NT), 0, S(), phys(), c->ix, (int)comp(),
+                                   I(), sI(), c->fs, dfr, 0, c->p.Nx, twy());
+            })
+        });
+        return HOS_OK;
     }
 };

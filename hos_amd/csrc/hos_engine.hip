@@ -183,8 +183,10 @@ static void layout(hos_ctx *c)
     c->f32 = p.precision == HOS_FP32;
     c->fs = (long)c->Nk * p.Nx;
     c->fsP = (long)p.Nx * p.Ny;
-    c->nI = 5 + c->M;
-    if (c->nI < 4) c->nI = 4;
+    // x-intermediate slots: all level-0 outputs (5 + M, the Zvel path), or the
+    // working slots 0 .. max(2, M)-1 plus the M + 4 fields the RHS pass
+    // transforms itself
+    c->nI = std::max(5 + c->M, std::max(2, c->M) + c->M + 4);
     HosIdx &ix = c->ix;
     std::memset(&ix, 0, sizeof ix);
     ix.M = c->M;

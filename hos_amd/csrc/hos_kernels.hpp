@@ -511,51 +511,6 @@ __device__ __forceinline__ F w_upto_v(const F *W)
     return s.value();
 }
 
-// nonlinear RHS in physical space (rhs_hos, hosm_2d_par.c:149-206) without
-// the two linear terms W^(1) and -g eta, which the x epilogue adds in FP64:
-//   N_eta = -(eta_x phi_x + eta_y phi_y) + (W^(M) - W^(1)) + |grad eta|^2 W^(M-2)
-//   N_phi = 1/2[(W^2)^(M) + |grad eta|^2 (W^2)^(M-2)] - 1/2 |grad phi|^2
-template <class T, int M>
-struct ProRhs {
-    Phys<T> ph;
-    HosIdx ix;
-    bool comp;
-    using F = vec_t<T>;
-    static constexpr int ND = M * (M + 1) / 2;
-    template <bool C>
-    __device__ __forceinline__ void calc(const F *f, F &a, F &b) const
-    {
-        const F e = f[0], ex = f[1], ey = f[2], px = f[3], py = f[4];
-        F W[M + 1];
-        w_orders_v<F, T, M, C>(ix, e, f + 5, W);
-        const F G = ex * ex + ey * ey;
-        F wM2 = F(0), w2M2 = F(0);
-        if constexpr (M - 2 >= 1) wM2 = w_upto_v<F, 1, M - 2, C>(W);
-        const F wMnl = w_upto_v<F, 2, M, C>(W);
-        a = -(ex * px + ey * py) + wMnl + G * wM2;
-        const F w2M = w2_upto_v<F, M, C>(W);
-        if constexpr (M - 2 >= 1) w2M2 = w2_upto_v<F, M - 2, C>(W);
-        b = T(0.5) * (w2M + G * w2M2) - T(0.5) * (px * px + py * py);
-    }
-    __device__ __forceinline__ void operator()(int m, int x, int y, F &a, F &b) const
-    {
-        F f[5 + ND];
-        f[0] = ph.vat(m, ix.eta, x, y);
-        f[1] = ph.vat(m, ix.etax, x, y);
-        f[2] = ph.vat(m, ix.etay, x, y);
-        f[3] = ph.vat(m, ix.phix, x, y);
-        f[4] = ph.vat(m, ix.phiy, x, y);
-#pragma unroll
-        for (int p = 1; p <= M; ++p)
-#pragma unroll
-            for (int j = 1; j <= M - p + 1; ++j) f[5 + d_pos<M>(p, j)] = ph.vat(m, ix.d[p][j], x, y);
-        if (comp)
-            calc<true>(f, a, b);
-        else
-            calc<false>(f, a, b);
-    }
-};
-
 // Zvel outputs: which=0 -> (W^(M), W^(M-2)), which=1 -> ((W^2)^(M), (W^2)^(M-2)),
 // order M at run time, one point at a time (the Zvel entry point, not the step)
 template <class T>
@@ -693,13 +648,216 @@ __global__ __launch_bounds__(NT) void k_yr2c_pair(PRO pro, cx<T> *__restrict__ I
 }
 
 // ---------------------------------------------------------------------------
+// The RHS pass with the deferred fields.  Nine of the physical fields (at M=5)
+// are read by nothing but the RHS prologue: the gradients eta_x, eta_y, phi_x,
+// phi_y and the M terms of W^(M), d^{k+1} phi^(M-k)/dz^{k+1}, k = 0..M-1 (the
+// fields with order + derivative = M+1).  Instead of a y c2r to the physical
+// grid and a read back, this kernel does their y c2r itself, one field after
+// the other in LDS (the next field's tile is loaded during the current FFT),
+// and keeps at each of its points only the four gradients and the compensated
+// sum W^(M).  The other fields (eta and d^j phi^(a) with a+j <= M) come from
+// the physical grid as before.  Then the nonlinear RHS in physical space
+// (rhs_hos, hosm_2d_par.c:149-206) without the two linear terms W^(1) and
+// -g eta, which the x epilogue adds in FP64:
+//   N_eta = -(eta_x phi_x + eta_y phi_y) + (W^(M) - W^(1)) + |grad eta|^2 W^(M-2)
+//   N_phi = 1/2[(W^2)^(M) + |grad eta|^2 (W^2)^(M-2)] - 1/2 |grad phi|^2
+// with W^(a) (hosm_2d_par.c:322-357) and the W sums (:362-442) as in
+// w_orders_v / w_upto_v / w2_upto_v, and their y r2c, written interleaved as
+// in k_yr2c_pair.  grid (Nx/TXP, 1, members)
+template <int M>
+struct RhsDeferred {
+    int slot[M + 4];  // x-intermediate slot of eta_x, eta_y, phi_x, phi_y, then W^(M) term k = 0..M-1
+};
+
+// y c2r tile load: the transposed tile of x-intermediate field src (x-rows
+// x0 .. x0+2 NPR-1) into registers, 16 consecutive ky of one row pair per 16
+// lanes (as c2r_tile)
+template <class T, int LOGNY, int NPR>
+struct C2rTile {
+    static constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1;
+    static constexpr int NL = ((NK + 15) / 16) * 16 * NPR, PERL = (NL + NT - 1) / NT;
+    cx<T> A[PERL], B[PERL];
+    __device__ __forceinline__ void load(const cx<T> *__restrict__ src, int x0, int nx)
+    {
+#pragma unroll
+        for (int q = 0; q < PERL; ++q) {
+            const int e = threadIdx.x + q * NT;
+            const int p = (e / 16) % NPR;
+            const int ky = min((e / (16 * NPR)) * 16 + e % 16, NK - 1);  // past the end: reload, unused
+            load2(src + (long)ky * nx + x0 + 2 * p, A[q], B[q]);
+        }
+    }
+    // Hermitian fill of the NPR complex rows in buf (Lds<T, NY> layout)
+    __device__ __forceinline__ void fill(cx<T> *buf) const
+    {
+        using L = Lds<T, NY>;
+#pragma unroll
+        for (int q = 0; q < PERL; ++q) {
+            const int e = threadIdx.x + q * NT;
+            const int p = (e / 16) % NPR, ky = (e / (16 * NPR)) * 16 + e % 16;
+            if (((NL % NT) == 0 || e < NL) && ky < NK) {
+                cx<T> a = A[q], b = B[q];
+                if (ky == 0 || ky == NY / 2) {
+                    a.y = T(0);
+                    b.y = T(0);
+                }
+                buf[L::at(p, ky)] = {a.x - b.y, a.y + b.x};
+                if (ky > 0 && ky < NY / 2) buf[L::at(p, NY - ky)] = {a.x + b.y, b.x - a.y};
+            }
+        }
+    }
+};
+
+template <class T, int LOGNY, int TXP, int M, bool C>
+__device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, const cx<T> *__restrict__ Im,
+                                         long fsI, const RhsDeferred<M> &dfr, cx<T> *__restrict__ out, int m,
+                                         int x0, int nx, cx<T> *buf, const cx<T> *__restrict__ twy)
+{
+    using F = vec_t<T>;
+    constexpr int V = Vec<T>::N;
+    constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1, NYV = NY / V, NPR = TXP / 2;
+    constexpr int NTASK = TXP * NYV, PER = (NTASK + NT - 1) / NT;
+    constexpr int ND = M + 4;
+    using L = Lds<T, NY>;
+    // this thread's tasks: V consecutive y of x-row p (clamped past the end)
+    int tp[PER], ty[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = min((int)threadIdx.x + q * NT, NTASK - 1);
+        tp[q] = e / NYV;
+        ty[q] = (e % NYV) * V;
+    }
+    F eta[PER], ex[PER], ey[PER], px[PER], py[PER];
+    KSum<F, C> wm[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) eta[q] = ph.vat(m, ix.eta, x0 + tp[q], ty[q]);
+    // phase A: y c2r of the deferred fields, BF at a time (BF*NPR rows share
+    // the stages and barriers of one LDS FFT); the next batch's tiles are
+    // loaded into the same registers during the current FFT
+    constexpr int BF = 3, NB = (ND + BF - 1) / BF;
+    C2rTile<T, LOGNY, NPR> tile[BF];
+#pragma unroll
+    for (int i = 0; i < BF; ++i)
+        if (i < ND) tile[i].load(Im + dfr.slot[i] * fsI, x0, nx);
+    for (int b = 0; b < NB; ++b) {
+        const int nf = min(BF, ND - b * BF);
+#pragma unroll
+        for (int i = 0; i < BF; ++i)
+            if (i < nf) tile[i].fill(buf + i * NPR * L::PITCH);
+        if (b + 1 < NB) {
+#pragma unroll
+            for (int i = 0; i < BF; ++i) {
+                const int f = (b + 1) * BF + i;
+                if (f < ND) tile[i].load(Im + dfr.slot[f] * fsI, x0, nx);
+            }
+        }
+        if (nf == 3)
+            lds_fft<T, LOGNY, 3 * NPR, NT, true>(buf, twy);
+        else if (nf == 2)
+            lds_fft<T, LOGNY, 2 * NPR, NT, true>(buf, twy);
+        else
+            lds_fft<T, LOGNY, NPR, NT, true>(buf, twy);
+#pragma unroll
+        for (int i = 0; i < BF; ++i) {
+            if (i >= nf) break;
+            const int f = b * BF + i;
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                F val;
+                const cx<T> *z = buf + L::at(i * NPR + (tp[q] >> 1), ty[q]);
+#pragma unroll
+                for (int v = 0; v < V; ++v) val[v] = (tp[q] & 1) ? z[v].y : z[v].x;
+                if (f == 0) ex[q] = val;
+                else if (f == 1) ey[q] = val;
+                else if (f == 2) px[q] = val;
+                else if (f == 3) py[q] = val;
+                else {
+                    // W^(M) term k = f-4: eta^k/k! d^{k+1} phi^(M-k)/dz^{k+1}, k ascending
+                    F ek = F(1);
+                    for (int k = 0; k < f - 4; ++k) ek *= eta[q];
+                    wm[q].add(ek * T(ix.invfact[f - 4]) * val);
+                }
+            }
+        }
+        __syncthreads();  // buf is refilled next
+    }
+    // phase B: N_eta, N_phi at the tasks' points
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = threadIdx.x + q * NT;
+        if ((NTASK % NT) != 0 && e >= NTASK) continue;
+        const int x = x0 + tp[q], y = ty[q];
+        constexpr int NF = (M - 1) * M / 2;  // d^j phi^(a) with a + j <= M
+        F d[NF > 0 ? NF : 1];
+#pragma unroll
+        for (int a = 1; a < M; ++a)
+#pragma unroll
+            for (int j = 1; a + j <= M; ++j) d[d_pos<M - 1>(a, j)] = ph.vat(m, ix.d[a][j], x, y);
+        F W[M + 1];
+#pragma unroll
+        for (int a = 1; a < M; ++a) {
+            F w = F(1);
+            KSum<F, C> sum;
+#pragma unroll
+            for (int k = 0; k < a; ++k) {
+                sum.add(w * T(ix.invfact[k]) * d[d_pos<M - 1>(a - k, k + 1)]);
+                w *= eta[q];
+            }
+            W[a] = sum.value();
+        }
+        W[M] = wm[q].value();
+        const F G = ex[q] * ex[q] + ey[q] * ey[q];
+        F wM2 = F(0), w2M2 = F(0);
+        if constexpr (M - 2 >= 1) wM2 = w_upto_v<F, 1, M - 2, C>(W);
+        const F wMnl = w_upto_v<F, 2, M, C>(W);
+        const F na = -(ex[q] * px[q] + ey[q] * py[q]) + wMnl + G * wM2;
+        const F w2M = w2_upto_v<F, M, C>(W);
+        if constexpr (M - 2 >= 1) w2M2 = w2_upto_v<F, M - 2, C>(W);
+        const F nb = T(0.5) * (w2M + G * w2M2) - T(0.5) * (px[q] * px[q] + py[q] * py[q]);
+        cx<T> *o = buf + L::at(tp[q], y);
+#pragma unroll
+        for (int v = 0; v < V; ++v) o[v] = {na[v], nb[v]};
+    }
+    lds_fft<T, LOGNY, TXP, NT, false>(buf, twy);
+    constexpr int NO = ((NK + 15) / 16) * 16 * TXP;
+    for (int e = threadIdx.x; e < NO; e += NT) {
+        const int p = (e / 16) % TXP, ky = (e / (16 * TXP)) * 16 + e % 16;
+        if (ky >= NK) continue;
+        const cx<T> zk = buf[L::at(p, ky)];
+        const cx<T> zm = buf[L::at(p, (NY - ky) & (NY - 1))];
+        store2(out + 2 * ((long)ky * nx + x0 + p), cx<T>{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)},
+               cx<T>{T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x)});
+    }
+}
+
+template <class T, int LOGNY, int TXP, int M>
+__global__ __launch_bounds__(NT) void k_yr2c_rhs(Phys<T> ph, HosIdx ix, int comp, cx<T> *__restrict__ I, long sI,
+                                                 long fsI, RhsDeferred<M> dfr, int dst, int nx,
+                                                 const cx<T> *__restrict__ twy)
+{
+    static_assert(TXP % 2 == 0, "x-rows in pairs");
+    constexpr int NY = 1 << LOGNY;
+    // the r2c of TXP rows, or a batch of 3 deferred fields of TXP/2 rows each
+    __shared__ cx<T> buf[Lds<T, NY>::size(TXP > 3 * (TXP / 2) ? TXP : 3 * (TXP / 2))];
+    const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TXP;
+    const int m = blockIdx.z;
+    const cx<T> *Im = I + m * sI;
+    cx<T> *out = I + m * sI + dst * fsI;
+    if (comp)
+        rhs_tile<T, LOGNY, TXP, M, true>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, twy);
+    else
+        rhs_tile<T, LOGNY, TXP, M, false>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, twy);
+}
+
+// ---------------------------------------------------------------------------
 // x forward pass of phi^(lev+1), then d^j/dz^j for j = 1..ndst and the x
-// inverse of each: the spectral phi^(lev+1) never leaves the chip (each
+// inverse of each, written to fields dst0 .. dst0+ndst-1 (the last one to
+// field dtop instead when dtop >= 0): the spectral phi^(lev+1) never leaves the chip (each
 // thread keeps its modes in registers; the inverses run XNB rows at a time).
 // grid (ceil(Nk/ROWS), 1, members)
 template <class T, int LOGNX, int ROWS>
 __global__ __launch_bounds__(NT) void k_xfwdinv(cx<T> *__restrict__ I, long sI, long fs, int src, int dst0, int ndst,
-                                                double scale, KTab kt, const cx<T> *__restrict__ twx)
+                                                int dtop, double scale, KTab kt, const cx<T> *__restrict__ twx)
 {
     using G = XGeom<T, LOGNX>;
     static_assert(G::ROWS == ROWS, "x rows");
@@ -756,7 +914,8 @@ __global__ __launch_bounds__(NT) void k_xfwdinv(cx<T> *__restrict__ I, long sI, 
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             if (j0 + b > ndst) break;
-            cx<T> *out = base + (dst0 + j0 + b - 1) * fs;
+            const int j = j0 + b;
+            cx<T> *out = base + ((j == ndst && dtop >= 0) ? dtop : dst0 + j - 1) * fs;
 #pragma unroll
             for (int q = 0; q < PER; ++q) {
                 const int e = threadIdx.x + q * NT;
@@ -1018,7 +1177,8 @@ __global__ __launch_bounds__(NT) void k_xinv_w(const cx<double> *__restrict__ Y,
 // each, one row per wave: the spectral row stays in registers throughout
 template <class T, int LOGNX>
 __global__ __launch_bounds__(NT) void k_xfwdinv_w(cx<T> *__restrict__ I, long sI, long fs, int src, int dst0,
-                                                  int ndst, double scale, KTab kt, const cx<T> *__restrict__ twx)
+                                                  int ndst, int dtop, double scale, KTab kt,
+                                                  const cx<T> *__restrict__ twx)
 {
     constexpr int NX = 1 << LOGNX, E = NX / 64;
     using L = Lds<T, NX>;
@@ -1062,7 +1222,7 @@ __global__ __launch_bounds__(NT) void k_xfwdinv_w(cx<T> *__restrict__ I, long sI
             v[i] = {mul * s[i].x, mul * s[i].y};
         }
         wave_fft<T, LOGNX, true>(v, row, twx, lane);
-        cx<T> *out = base + (dst0 + j - 1) * fs + r0;
+        cx<T> *out = base + ((j == ndst && dtop >= 0) ? dtop : dst0 + j - 1) * fs + r0;
 #pragma unroll
         for (int i = 0; i < E; ++i) out[lane + 64 * i] = v[i];
     }

@@ -1,5 +1,6 @@
 // hos_levels.hip -- the y r2c launches whose prologue is instantiated per
-// HOS level (phi^(lev+1), lev = 1..7) or per order (the RHS pair, M = 1..8).
+// HOS level (phi^(lev+1), lev = 1..7) or per order (the RHS pass
+// k_yr2c_rhs, M = 1..8).
 // Compiled once per precision (HOS_T) and part (HOS_PART 0: levels, 1: RHS)
 // by the Makefile, so the several hundred kernel instantiations build in
 // parallel translation units.
@@ -37,8 +38,7 @@ template int Eng<HOS_T>::phi_level(int);
 template <class T, int MO>
 static int rhs_pair_t(Eng<T> &e)
 {
-    ProRhs<T, MO> pro{e.phys(), e.c->ix, e.comp()};
-    return e.yr2c_pair(pro, 0, e.c->nP);
+    return e.template yr2c_rhs<MO>();
 }
 
 template <class T>
