@@ -430,7 +430,7 @@ struct Eng {
     }
 
     // the RHS pass of order MO (k_yr2c_rhs); TXP = half the pair tile, at most
-    // what keeps four 16-byte tasks per thread
+    // what keeps two 16-byte tasks per thread
     template <int MO>
     int yr2c_rhs()
     {
@@ -442,11 +442,11 @@ struct Eng {
             TX_SWITCH(T, tx, {
                 constexpr int V = Vec<T>::N;
                 constexpr int TXP0 = TXV / 2;
-                constexpr int TXP = (TXP0 > 2 && TXP0 * (1 << LG) / V > 4 * NT) ? 2 : TXP0;
+                constexpr int TXP = (TXP0 > 2 && TXP0 * (1 << LG) / V > 2 * NT_RHS) ? 2 : TXP0;
                 dim3 grid(c->p.Nx / TXP, 1, nm);
                 Prof pf(c, S(), HOS_K_YR2C_PAIR,
                         (double)nm * sizeof(T) * (nin * (double)c->fsP + 2.0 * (MO + 4) * c->fs + 4.0 * c->fs));
-                hipLaunchKernelGGL((k_yr2c_rhs<T, LG, TXP, MO>), grid, dim3(NT), 0, S(), phys(), c->ix, (int)comp(),
+                hipLaunchKernelGGL((k_yr2c_rhs<T, LG, TXP, MO>), grid, dim3(NT_RHS), 0, S(), phys(), c->ix, (int)comp(),
                                    I(), sI(), c->fs, dfr, 0, c->p.Nx, twy());
             })
         });

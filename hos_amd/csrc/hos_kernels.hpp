@@ -672,16 +672,16 @@ struct RhsDeferred {
 // y c2r tile load: the transposed tile of x-intermediate field src (x-rows
 // x0 .. x0+2 NPR-1) into registers, 16 consecutive ky of one row pair per 16
 // lanes (as c2r_tile)
-template <class T, int LOGNY, int NPR>
+template <class T, int LOGNY, int NPR, int NTT = NT>
 struct C2rTile {
     static constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1;
-    static constexpr int NL = ((NK + 15) / 16) * 16 * NPR, PERL = (NL + NT - 1) / NT;
+    static constexpr int NL = ((NK + 15) / 16) * 16 * NPR, PERL = (NL + NTT - 1) / NTT;
     cx<T> A[PERL], B[PERL];
     __device__ __forceinline__ void load(const cx<T> *__restrict__ src, int x0, int nx)
     {
 #pragma unroll
         for (int q = 0; q < PERL; ++q) {
-            const int e = threadIdx.x + q * NT;
+            const int e = threadIdx.x + q * NTT;
             const int p = (e / 16) % NPR;
             const int ky = min((e / (16 * NPR)) * 16 + e % 16, NK - 1);  // past the end: reload, unused
             load2(src + (long)ky * nx + x0 + 2 * p, A[q], B[q]);
@@ -693,9 +693,9 @@ struct C2rTile {
         using L = Lds<T, NY>;
 #pragma unroll
         for (int q = 0; q < PERL; ++q) {
-            const int e = threadIdx.x + q * NT;
+            const int e = threadIdx.x + q * NTT;
             const int p = (e / 16) % NPR, ky = (e / (16 * NPR)) * 16 + e % 16;
-            if (((NL % NT) == 0 || e < NL) && ky < NK) {
+            if (((NL % NTT) == 0 || e < NL) && ky < NK) {
                 cx<T> a = A[q], b = B[q];
                 if (ky == 0 || ky == NY / 2) {
                     a.y = T(0);
@@ -708,7 +708,7 @@ struct C2rTile {
     }
 };
 
-template <class T, int LOGNY, int TXP, int M, bool C>
+template <class T, int LOGNY, int TXP, int M, bool C, int NTT>
 __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, const cx<T> *__restrict__ Im,
                                          long fsI, const RhsDeferred<M> &dfr, cx<T> *__restrict__ out, int m,
                                          int x0, int nx, cx<T> *buf, const cx<T> *__restrict__ twy)
@@ -716,14 +716,14 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
     using F = vec_t<T>;
     constexpr int V = Vec<T>::N;
     constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1, NYV = NY / V, NPR = TXP / 2;
-    constexpr int NTASK = TXP * NYV, PER = (NTASK + NT - 1) / NT;
+    constexpr int NTASK = TXP * NYV, PER = (NTASK + NTT - 1) / NTT;
     constexpr int ND = M + 4;
     using L = Lds<T, NY>;
     // this thread's tasks: V consecutive y of x-row p (clamped past the end)
     int tp[PER], ty[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        const int e = min((int)threadIdx.x + q * NT, NTASK - 1);
+        const int e = min((int)threadIdx.x + q * NTT, NTASK - 1);
         tp[q] = e / NYV;
         ty[q] = (e % NYV) * V;
     }
@@ -735,7 +735,7 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
     // the stages and barriers of one LDS FFT); the next batch's tiles are
     // loaded into the same registers during the current FFT
     constexpr int BF = 3, NB = (ND + BF - 1) / BF;
-    C2rTile<T, LOGNY, NPR> tile[BF];
+    C2rTile<T, LOGNY, NPR, NTT> tile[BF];
 #pragma unroll
     for (int i = 0; i < BF; ++i)
         if (i < ND) tile[i].load(Im + dfr.slot[i] * fsI, x0, nx);
@@ -752,11 +752,11 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
             }
         }
         if (nf == 3)
-            lds_fft<T, LOGNY, 3 * NPR, NT, true>(buf, twy);
+            lds_fft<T, LOGNY, 3 * NPR, NTT, true>(buf, twy);
         else if (nf == 2)
-            lds_fft<T, LOGNY, 2 * NPR, NT, true>(buf, twy);
+            lds_fft<T, LOGNY, 2 * NPR, NTT, true>(buf, twy);
         else
-            lds_fft<T, LOGNY, NPR, NT, true>(buf, twy);
+            lds_fft<T, LOGNY, NPR, NTT, true>(buf, twy);
 #pragma unroll
         for (int i = 0; i < BF; ++i) {
             if (i >= nf) break;
@@ -784,8 +784,8 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
     // phase B: N_eta, N_phi at the tasks' points
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        const int e = threadIdx.x + q * NT;
-        if ((NTASK % NT) != 0 && e >= NTASK) continue;
+        const int e = threadIdx.x + q * NTT;
+        if ((NTASK % NTT) != 0 && e >= NTASK) continue;
         const int x = x0 + tp[q], y = ty[q];
         constexpr int NF = (M - 1) * M / 2;  // d^j phi^(a) with a + j <= M
         F d[NF > 0 ? NF : 1];
@@ -818,9 +818,9 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
 #pragma unroll
         for (int v = 0; v < V; ++v) o[v] = {na[v], nb[v]};
     }
-    lds_fft<T, LOGNY, TXP, NT, false>(buf, twy);
+    lds_fft<T, LOGNY, TXP, NTT, false>(buf, twy);
     constexpr int NO = ((NK + 15) / 16) * 16 * TXP;
-    for (int e = threadIdx.x; e < NO; e += NT) {
+    for (int e = threadIdx.x; e < NO; e += NTT) {
         const int p = (e / 16) % TXP, ky = (e / (16 * TXP)) * 16 + e % 16;
         if (ky >= NK) continue;
         const cx<T> zk = buf[L::at(p, ky)];
@@ -830,8 +830,12 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
     }
 }
 
+// 512 threads: half the per-thread points (and accumulators) of a 256-thread
+// workgroup on the same tile
+constexpr int NT_RHS = 512;
+
 template <class T, int LOGNY, int TXP, int M>
-__global__ __launch_bounds__(NT) void k_yr2c_rhs(Phys<T> ph, HosIdx ix, int comp, cx<T> *__restrict__ I, long sI,
+__global__ __launch_bounds__(NT_RHS) void k_yr2c_rhs(Phys<T> ph, HosIdx ix, int comp, cx<T> *__restrict__ I, long sI,
                                                  long fsI, RhsDeferred<M> dfr, int dst, int nx,
                                                  const cx<T> *__restrict__ twy)
 {
@@ -844,9 +848,9 @@ __global__ __launch_bounds__(NT) void k_yr2c_rhs(Phys<T> ph, HosIdx ix, int comp
     const cx<T> *Im = I + m * sI;
     cx<T> *out = I + m * sI + dst * fsI;
     if (comp)
-        rhs_tile<T, LOGNY, TXP, M, true>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, twy);
+        rhs_tile<T, LOGNY, TXP, M, true, NT_RHS>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, twy);
     else
-        rhs_tile<T, LOGNY, TXP, M, false>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, twy);
+        rhs_tile<T, LOGNY, TXP, M, false, NT_RHS>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, twy);
 }
 
 // ---------------------------------------------------------------------------
