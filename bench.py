@@ -8,14 +8,20 @@ spreading, |k| <= 4 kp, seed 1, classical RK4 with dt = Tp/100.
 
 A step is one RK4 step (4 RHS evaluations) of every member.  Timed in order:
   1. the FP32 batched form: --members (default 8) realisations of the
-     workload advanced in the same launches (FFTs, spectral multiplies and
-     products in FP32 with Kahan-compensated order sums; FP64 state, stage
-     combination, linear terms and diagnostics), one captured graph per step;
+     workload advanced together (FFTs, spectral multiplies and products in
+     FP32 with Kahan-compensated order sums; FP64 state, stage combination,
+     linear terms and diagnostics), launch shapes chosen by the autotuner
+     (y tiles, x-pass mode, one chain or two member groups on their own
+     streams), captured graphs;
   2. the same FP32 form on one instance;
   3. the plain FP64 form on one instance, same input.
 value = grid-point RK4 steps/s of (1): members * Nx * Ny * steps / wall time
 of the timed steps, state resident in HBM, between two device
 synchronisations.  p50/p99 are per-step times from HIP events.
+roofline: a separate stream-ordered pass of the same batch (one chain, so
+kernels do not overlap) with a HIP event pair around every kernel on the
+solver's stream; the dominant kernel's algorithmic bytes per launch over its
+mean duration.
 
 Prints one JSON line.
 """
@@ -48,15 +54,26 @@ def make_state(hos, w):
                        depth=w["depth"], g=w["g"], seed=w["seed"], nlevs=w["nlevs"])
 
 
-def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_steps=0, autotune=True):
+def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_steps=0, autotune=True, tiles=None):
     p = hos.params(Nx=w["Nx"], Ny=w["Ny"], nlevs=w["nlevs"], Lx=w["Lx"], Ly=w["Ly"], g=w["g"],
                    depth=w["depth"], precision=prec, compensated=1)
     s = hos.Solver(p, members)
     for m in range(members):
         s.set_state(u0, m)
         s.set_member(m, w["depth"], dt)
-    tiles = s.autotune(3) if autotune else None
+    if tiles:
+        # fixed launch shapes (c2r, r2c, pair y tiles; x-pass mode), no tuning trials
+        for kind, v in enumerate(tiles[:3]):
+            s.set_tile(v, kind)
+        if len(tiles) > 3:
+            s.set_tile(tiles[3], hos.HOS_TILE_XPASS)
+        if len(tiles) > 4:
+            s.set_tile(tiles[4], hos.HOS_TILE_GROUPS)
+        tiles = tuple(tiles[:3])
+    else:
+        tiles = s.autotune(3) if autotune else None
     xpass = s.get_tile(hos.HOS_TILE_XPASS)
+    groups = s.get_tile(hos.HOS_TILE_GROUPS)
     s.use_graph(graph)
     s.step(warmup, dt)
     s.sync()
@@ -77,6 +94,7 @@ def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_s
         "bytes_per_pt_step": hos.bytes_per_point_step(p),
         "tiles": tiles,
         "xpass": "row per wave" if xpass == 2 else "rows per workgroup",
+        "groups": groups,
     }
     if profile_steps:
         # same workload, stream-ordered launches with a HIP event pair around
@@ -113,7 +131,9 @@ def cpu_baseline(w, dt, u0):
     }
 
 
-KIND_OF = [("k_yr2c_pair", "yr2c_pair"), ("k_yr2c", "yr2c"), ("k_yc2r", "yc2r"), ("k_xfwdinv", "xfwdinv"),
+# kernel name -> profiled kind (hos_kernel_stats): the RHS pass k_yr2c_rhs is
+# the "yr2c_pair" kind (it replaced the plain pair pass in the step)
+KIND_OF = [("k_yr2c_pair", "yr2c_pair"), ("k_yr2c_rhs", "yr2c_pair"), ("k_yr2c", "yr2c"), ("k_yc2r", "yc2r"), ("k_xfwdinv", "xfwdinv"),
            ("k_xfwd_rhs", "xfwd_rhs"), ("k_xinv", "xinv"), ("k_copy_c64", "copy")]
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_1024_m5_x8.json")
 
@@ -161,6 +181,9 @@ def main():
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--tiles", type=str, default=None,
+                    help="fixed launch shapes c2r,r2c,pair[,xpass[,groups]] for the batched form instead of the autotuner "
+                         "(profiler runs: no tuning trials in the kernel statistics)")
     ap.add_argument("--calib", action="store_true", help="only the plain copy kernel (counter calibration)")
     ap.add_argument("--traffic", nargs=4, metavar=("FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
                     help="summarise four rocprofv3 --pmc output folders into " + TRAFFIC_FILE)
@@ -192,8 +215,9 @@ def main():
     print(f"[bench] device: {hos.device_name()}", file=sys.stderr, flush=True)
     graph = not args.no_graph
 
+    fixed = [int(v) for v in args.tiles.split(",")] if args.tiles else None
     f32 = run_form(hos, w, hos.HOS_FP32, args.members, args.steps, args.warmup, dt, u0, graph=graph,
-                   profile_steps=args.profile_steps)
+                   profile_steps=args.profile_steps, tiles=fixed)
     print(f"[bench] fp32 x{args.members}: {f32['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     f1 = None
     if not args.no_single and args.members != 1:
@@ -255,7 +279,7 @@ def main():
                         "wavelengths, kp*h=1, M=5, JONSWAP gamma=5 kp*Hs/2=0.12 seed 1, RK4 dt=Tp/100, all "
                         "members in the same launches; compared with the plain FP64 single instance",
             "members": args.members, "graph": graph, "y_tiles_c2r_r2c_pair": f32["tiles"],
-            "x_inverse_passes": f32["xpass"],
+            "x_inverse_passes": f32["xpass"], "member_groups": f32["groups"],
             "form": "FP32 FFTs/products with Kahan order sums; FP64 state, RK combination, linear terms",
             "parallelism": "single GPU",
         },
