@@ -823,14 +823,19 @@ extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
     double best_ms = 1e30;
     const int saved_graph = c->use_graph;
     c->use_graph = 1;
-    // one timed trial of the current launch shapes
+    // one trial of the current launch shapes: capture and warm up, then the
+    // faster of two timed runs of nsteps (one run is too noisy to rank shapes
+    // a few percent apart)
     auto trial = [&](double &ms) {
         drop_graphs(c);
-        int r = step_enqueue(c, 1);  // capture + warm
+        int r = step_enqueue(c, 1);
         if (!r) r = step_finish(c);
-        if (!r) r = step_enqueue(c, nsteps);
-        if (!r) r = step_finish(c);
-        ms = c->last_ms;
+        ms = 1e30;
+        for (int rep = 0; rep < 2 && !r; ++rep) {
+            r = step_enqueue(c, nsteps);
+            if (!r) r = step_finish(c);
+            if (!r) ms = std::min(ms, c->last_ms);
+        }
         return r;
     };
     for (int a = 0; a < 3 && !rc; ++a)

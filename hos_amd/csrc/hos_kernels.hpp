@@ -708,6 +708,70 @@ struct C2rTile {
     }
 };
 
+// phase B of the RHS pass: N_eta, N_phi at the tasks' points from the physical
+// fields (eta and d^j phi^(a), a + j <= M) and the deferred values, then the y
+// r2c and the interleaved store
+template <class T, int LOGNY, int TXP, int M, bool C, int NTT, int PER>
+__device__ __forceinline__ void rhs_phase_b(const Phys<T> &ph, const HosIdx &ix, cx<T> *__restrict__ out, int m,
+                                            int x0, int nx, cx<T> *buf, const cx<T> *__restrict__ twy,
+                                            const int *tp, const int *ty, const vec_t<T> *eta, const vec_t<T> *ex,
+                                            const vec_t<T> *ey, const vec_t<T> *px, const vec_t<T> *py,
+                                            const KSum<vec_t<T>, C> *wm)
+{
+    using F = vec_t<T>;
+    constexpr int V = Vec<T>::N;
+    constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1, NYV = NY / V;
+    constexpr int NTASK = TXP * NYV;
+    using L = Lds<T, NY>;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = threadIdx.x + q * NTT;
+        if ((NTASK % NTT) != 0 && e >= NTASK) continue;
+        const int x = x0 + tp[q], y = ty[q];
+        constexpr int NF = (M - 1) * M / 2;  // d^j phi^(a) with a + j <= M
+        F d[NF > 0 ? NF : 1];
+#pragma unroll
+        for (int a = 1; a < M; ++a)
+#pragma unroll
+            for (int j = 1; a + j <= M; ++j) d[d_pos<M - 1>(a, j)] = ph.vat(m, ix.d[a][j], x, y);
+        F W[M + 1];
+#pragma unroll
+        for (int a = 1; a < M; ++a) {
+            F w = F(1);
+            KSum<F, C> sum;
+#pragma unroll
+            for (int k = 0; k < a; ++k) {
+                sum.add(w * T(ix.invfact[k]) * d[d_pos<M - 1>(a - k, k + 1)]);
+                w *= eta[q];
+            }
+            W[a] = sum.value();
+        }
+        W[M] = wm[q].value();
+        const F G = ex[q] * ex[q] + ey[q] * ey[q];
+        F wM2 = F(0), w2M2 = F(0);
+        if constexpr (M - 2 >= 1) wM2 = w_upto_v<F, 1, M - 2, C>(W);
+        const F wMnl = w_upto_v<F, 2, M, C>(W);
+        const F na = -(ex[q] * px[q] + ey[q] * py[q]) + wMnl + G * wM2;
+        const F w2M = w2_upto_v<F, M, C>(W);
+        if constexpr (M - 2 >= 1) w2M2 = w2_upto_v<F, M - 2, C>(W);
+        const F nb = T(0.5) * (w2M + G * w2M2) - T(0.5) * (px[q] * px[q] + py[q] * py[q]);
+        cx<T> *o = buf + L::at(tp[q], y);
+#pragma unroll
+        for (int v = 0; v < V; ++v) o[v] = {na[v], nb[v]};
+    }
+    lds_fft<T, LOGNY, TXP, NTT, false>(buf, twy);
+    constexpr int NO = ((NK + 15) / 16) * 16 * TXP;
+    for (int e = threadIdx.x; e < NO; e += NTT) {
+        const int p = (e / 16) % TXP, ky = (e / (16 * TXP)) * 16 + e % 16;
+        if (ky >= NK) continue;
+        const cx<T> zk = buf[L::at(p, ky)];
+        const cx<T> zm = buf[L::at(p, (NY - ky) & (NY - 1))];
+        store2(out + 2 * ((long)ky * nx + x0 + p), cx<T>{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)},
+               cx<T>{T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x)});
+    }
+}
+
+
 template <class T, int LOGNY, int TXP, int M, bool C, int NTT>
 __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, const cx<T> *__restrict__ Im,
                                          long fsI, const RhsDeferred<M> &dfr, cx<T> *__restrict__ out, int m,
@@ -715,7 +779,7 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
 {
     using F = vec_t<T>;
     constexpr int V = Vec<T>::N;
-    constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1, NYV = NY / V, NPR = TXP / 2;
+    constexpr int NY = 1 << LOGNY, NYV = NY / V, NPR = TXP / 2;
     constexpr int NTASK = TXP * NYV, PER = (NTASK + NTT - 1) / NTT;
     constexpr int ND = M + 4;
     using L = Lds<T, NY>;
@@ -781,53 +845,7 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
         }
         __syncthreads();  // buf is refilled next
     }
-    // phase B: N_eta, N_phi at the tasks' points
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int e = threadIdx.x + q * NTT;
-        if ((NTASK % NTT) != 0 && e >= NTASK) continue;
-        const int x = x0 + tp[q], y = ty[q];
-        constexpr int NF = (M - 1) * M / 2;  // d^j phi^(a) with a + j <= M
-        F d[NF > 0 ? NF : 1];
-#pragma unroll
-        for (int a = 1; a < M; ++a)
-#pragma unroll
-            for (int j = 1; a + j <= M; ++j) d[d_pos<M - 1>(a, j)] = ph.vat(m, ix.d[a][j], x, y);
-        F W[M + 1];
-#pragma unroll
-        for (int a = 1; a < M; ++a) {
-            F w = F(1);
-            KSum<F, C> sum;
-#pragma unroll
-            for (int k = 0; k < a; ++k) {
-                sum.add(w * T(ix.invfact[k]) * d[d_pos<M - 1>(a - k, k + 1)]);
-                w *= eta[q];
-            }
-            W[a] = sum.value();
-        }
-        W[M] = wm[q].value();
-        const F G = ex[q] * ex[q] + ey[q] * ey[q];
-        F wM2 = F(0), w2M2 = F(0);
-        if constexpr (M - 2 >= 1) wM2 = w_upto_v<F, 1, M - 2, C>(W);
-        const F wMnl = w_upto_v<F, 2, M, C>(W);
-        const F na = -(ex[q] * px[q] + ey[q] * py[q]) + wMnl + G * wM2;
-        const F w2M = w2_upto_v<F, M, C>(W);
-        if constexpr (M - 2 >= 1) w2M2 = w2_upto_v<F, M - 2, C>(W);
-        const F nb = T(0.5) * (w2M + G * w2M2) - T(0.5) * (px[q] * px[q] + py[q] * py[q]);
-        cx<T> *o = buf + L::at(tp[q], y);
-#pragma unroll
-        for (int v = 0; v < V; ++v) o[v] = {na[v], nb[v]};
-    }
-    lds_fft<T, LOGNY, TXP, NTT, false>(buf, twy);
-    constexpr int NO = ((NK + 15) / 16) * 16 * TXP;
-    for (int e = threadIdx.x; e < NO; e += NTT) {
-        const int p = (e / 16) % TXP, ky = (e / (16 * TXP)) * 16 + e % 16;
-        if (ky >= NK) continue;
-        const cx<T> zk = buf[L::at(p, ky)];
-        const cx<T> zm = buf[L::at(p, (NY - ky) & (NY - 1))];
-        store2(out + 2 * ((long)ky * nx + x0 + p), cx<T>{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)},
-               cx<T>{T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x)});
-    }
+    rhs_phase_b<T, LOGNY, TXP, M, C, NTT, PER>(ph, ix, out, m, x0, nx, buf, twy, tp, ty, eta, ex, ey, px, py, wm);
 }
 
 // 512 threads: half the per-thread points (and accumulators) of a 256-thread
@@ -836,8 +854,8 @@ constexpr int NT_RHS = 512;
 
 template <class T, int LOGNY, int TXP, int M>
 __global__ __launch_bounds__(NT_RHS) void k_yr2c_rhs(Phys<T> ph, HosIdx ix, int comp, cx<T> *__restrict__ I, long sI,
-                                                 long fsI, RhsDeferred<M> dfr, int dst, int nx,
-                                                 const cx<T> *__restrict__ twy)
+                                                     long fsI, RhsDeferred<M> dfr, int dst, int nx,
+                                                     const cx<T> *__restrict__ twy)
 {
     static_assert(TXP % 2 == 0, "x-rows in pairs");
     constexpr int NY = 1 << LOGNY;
