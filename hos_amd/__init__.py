@@ -442,7 +442,7 @@ class Solver:
         """y-pass tile (4, 8, 16; 0 = default) for kind HOS_TILE_C2R (0),
         HOS_TILE_R2C (1), HOS_TILE_PAIR (2) or all (3); for HOS_TILE_XPASS (4)
         the x-pass mode (1 rows per workgroup, 2 row per wave, 0 default); for
-        HOS_TILE_GROUPS (5) the member groups (1 or 2, 0 default)."""
+        HOS_TILE_GROUPS (5) the member groups (1..4, 0 default)."""
         _check(self.lib.hos_set_tile(self.h, kind, tx))
 
     def get_tile(self, kind: int) -> int:

@@ -30,12 +30,13 @@ struct hos_ctx {
     int nI = 0, nP = 0;
     int tx[3] = {8, 4, 4};  // y tiles: c2r, r2c, pair
     int xwave = 0;          // x inverse passes one row per wave (where available)
-    // member groups: with 2, members [0, n0) and [n0, active) run as two
-    // independent launch chains on their own streams (gst), forked from and
-    // joined to the context stream each step
+    // member groups: with G > 1, the active members split into G nearly equal
+    // ranges that run as independent launch chains on their own streams (gst),
+    // forked from and joined to the context stream each step
+    static constexpr int MAXG = 4;
     int ngroups = 1;
-    hipStream_t gst[2] = {nullptr, nullptr};
-    hipEvent_t gfork = nullptr, gjoin[2] = {nullptr, nullptr};
+    hipStream_t gst[MAXG] = {};
+    hipEvent_t gfork = nullptr, gjoin[MAXG] = {};
     bool f32 = false;
     long fs = 0;    // complex elements per spectral field  (Nk*Nx)
     long fsP = 0;   // real elements per physical field     (Nx*Ny)

@@ -150,9 +150,12 @@ static int default_tx(bool f32, int logny, int kind)
 // member runs faster with rows per workgroup)
 static int default_xwave(const hos_ctx *c) { return x_wave_ok(c) && c->nm >= 4 ? 1 : 0; }
 
+// slots per active count in the graph cache (group counts 1..MAXG)
+constexpr int GSLOTS = hos_ctx::MAXG * (hos_ctx::MAXG + 1) / 2;
+
 // two member groups on their own streams by default from 4 members (measured
-// at 8 x 1024^2: 6% faster than one chain; kernel tails of one chain overlap
-// the other's work)
+// at 8 x 1024^2: faster than one chain; the latency-bound phases of one chain
+// overlap the other's memory-bound passes)
 static int default_groups(const hos_ctx *c) { return c->nm >= 4 ? 2 : 1; }
 
 static int validate(const hos_params *p, int nm)
@@ -259,7 +262,7 @@ static void release(hos_ctx *c)
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
-    for (int g = 0; g < 2; ++g) {
+    for (int g = 0; g < hos_ctx::MAXG; ++g) {
         if (c->gst[g]) (void)hipStreamDestroy(c->gst[g]);
         if (c->gjoin[g]) (void)hipEventDestroy(c->gjoin[g]);
     }
@@ -293,7 +296,7 @@ extern "C" int hos_create(const hos_params *p, int nmembers, hos_ctx **out)
     };
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(HOS_E_DEVICE, "hipStreamCreate failed"));
-    for (int g = 0; g < 2; ++g) {
+    for (int g = 0; g < hos_ctx::MAXG; ++g) {
         if (hipStreamCreateWithFlags(&c->gst[g], hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&c->gjoin[g], hipEventDisableTiming) != hipSuccess)
             return bail(fail(HOS_E_DEVICE, "group stream set-up failed"));
@@ -357,8 +360,8 @@ extern "C" int hos_create(const hos_params *p, int nmembers, hos_ctx **out)
         for (int m = 0; m < nm; ++m)
             if ((rc = th_table(c, m))) return bail(rc);
     }
-    c->gexec_by_active.assign(3 * (nm + 1), nullptr);
-    c->graph_by_active.assign(3 * (nm + 1), nullptr);
+    c->gexec_by_active.assign(GSLOTS * (nm + 1), nullptr);
+    c->graph_by_active.assign(GSLOTS * (nm + 1), nullptr);
     if (hipDeviceSynchronize() != hipSuccess) return bail(fail(HOS_E_DEVICE, "device sync failed"));
     *out = c;
     return HOS_OK;
@@ -398,20 +401,28 @@ static RkArgs rk_base(hos_ctx *c)
     return rk;
 }
 
-// groups this step runs in (1 or 2); the generic Butcher path and the
-// per-kernel profile run as one chain
+// groups this step runs in: the setting, while every group keeps at least
+// two members; the generic Butcher path and the per-kernel profile run as one
+// chain
 static int groups_now(const hos_ctx *c)
 {
-    return (c->ngroups == 2 && c->subdiag && c->active >= 4 && !c->prof_on) ? 2 : 1;
+    if (!c->subdiag || c->prof_on) return 1;
+    int ng = c->ngroups;
+    while (ng > 1 && c->active < 2 * ng) --ng;
+    return ng;
 }
-// graph slot of (active count, group count, group)
-static size_t graph_slot(const hos_ctx *c, int ng, int g) { return 3 * (size_t)c->active + (ng == 1 ? 0 : 1 + g); }
+// graph slot of (active count, group count, group): groups 1..MAXG take
+// 1 + 2 + .. + MAXG slots per active count
+static size_t graph_slot(const hos_ctx *c, int ng, int g)
+{
+    return GSLOTS * (size_t)c->active + (size_t)(ng * (ng - 1) / 2 + g);
+}
 
+// members [m0, m0+n) of group g of ng: nearly equal ranges in order
 static void group_range(const hos_ctx *c, int g, int ng, int *m0, int *n)
 {
-    const int n0 = ng == 1 ? c->active : (c->active + 1) / 2;
-    *m0 = g == 0 ? 0 : n0;
-    *n = g == 0 ? n0 : c->active - n0;
+    *m0 = (int)((long)c->active * g / ng);
+    *n = (int)((long)c->active * (g + 1) / ng) - *m0;
 }
 
 // one RK step of members m0 .. m0+n-1, launched on stream st
@@ -741,7 +752,7 @@ extern "C" int hos_set_tile(hos_ctx *c, int kind, int tx)
 {
     if (!c) return fail(HOS_E_INVALID, "null context");
     if (kind == HOS_TILE_GROUPS) {
-        if (tx < 0 || tx > 2) return fail(HOS_E_INVALID, "member groups %d: 0 default, 1 or 2", tx);
+        if (tx < 0 || tx > hos_ctx::MAXG) return fail(HOS_E_INVALID, "member groups %d: 0 default, 1 .. %d", tx, hos_ctx::MAXG);
         if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
         c->ngroups = tx == 0 ? default_groups(c) : tx;
         drop_graphs(c);
@@ -864,16 +875,21 @@ extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
             best[3] = c->xwave;
         }
     }
-    if (!rc && c->subdiag && c->active >= 4) {
-        // the other member-group count with the best tiles and x-pass mode
+    if (!rc && c->subdiag) {
+        // the other member-group counts (each group at least two members)
+        // with the best tiles and x-pass mode
         for (int k = 0; k < 3; ++k) c->tx[k] = best[k];
         c->xwave = best[3];
-        c->ngroups = best[4] == 2 ? 1 : 2;
-        double ms = 0.0;
-        rc = trial(ms);
-        if (!rc && ms < best_ms) {
-            best_ms = ms;
-            best[4] = c->ngroups;
+        const int tried = best[4];
+        for (int ng = 1; ng <= hos_ctx::MAXG && !rc; ++ng) {
+            if (ng == tried || (ng > 1 && c->active < 2 * ng)) continue;
+            c->ngroups = ng;
+            double ms = 0.0;
+            rc = trial(ms);
+            if (!rc && ms < best_ms) {
+                best_ms = ms;
+                best[4] = ng;
+            }
         }
     }
     c->use_graph = saved_graph;

@@ -195,18 +195,19 @@ int hos_use_graph(hos_ctx *c, int enable);
  * default) for one kind of y pass or for all of them.  Kind HOS_TILE_XPASS:
  * the x inverse passes run rows per workgroup (1) or one row per 64-lane
  * wave (2, FP32 form with Nx = 512 or 1024); 0 = built-in default.  Kind
- * HOS_TILE_GROUPS: the active members step as one launch chain (1) or as two
- * halves, each its own chain of launches (and graph) on its own stream,
- * forked from and joined to the context stream every step (2: applies from 4
- * active members with the RK schemes 2-5); 0 = built-in default. */
+ * HOS_TILE_GROUPS: the active members step as one launch chain (1) or as G =
+ * 2..4 nearly equal ranges, each its own chain of launches (and graph) on its
+ * own stream, forked from and joined to the context stream every step (with
+ * the RK schemes 2-5, and only while every range keeps two members or more);
+ * 0 = built-in default. */
 #define HOS_TILE_C2R 0   /* k_yc2r: several fields per launch        */
 #define HOS_TILE_R2C 1   /* k_yr2c: one field per launch              */
 #define HOS_TILE_PAIR 2  /* k_yr2c_pair: the two RHS fields            */
 #define HOS_TILE_ALL 3
 #define HOS_TILE_XPASS 4 /* k_xinv, k_xfwdinv: workgroup rows or wave rows */
-#define HOS_TILE_GROUPS 5 /* member groups: 1 chain, or 2 on their own streams */
+#define HOS_TILE_GROUPS 5 /* member groups: 1 chain, or 2..4 on their own streams */
 int hos_set_tile(hos_ctx *c, int kind, int tx);
-/* current setting of one kind (0-2: tx; HOS_TILE_XPASS, HOS_TILE_GROUPS: 1 or 2), < 0 on error */
+/* current setting of one kind (0-2: tx; HOS_TILE_XPASS: 1 or 2; HOS_TILE_GROUPS: 1..4), < 0 on error */
 int hos_get_tile(hos_ctx *c, int kind);
 
 /* Autotuner: time nsteps RK steps of the active members for every valid

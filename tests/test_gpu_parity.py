@@ -168,16 +168,16 @@ def test_members_match_alone(hos):
 
 @pytest.mark.parametrize("graph", [False, True])
 def test_member_groups_match_one_chain(hos, graph):
-    """Two member groups on their own streams give bit-identical states and
-    energies to one launch chain, for an odd member count, and after the
-    active count drops below the two-group threshold and comes back."""
-    Nx, Ny, nlevs, nm = 64, 32, 3, 5
+    """Member groups on their own streams (2, 3, 4) give bit-identical states
+    and energies to one launch chain, for uneven ranges, and after the active
+    count drops (fewer groups) and comes back."""
+    Nx, Ny, nlevs, nm = 64, 32, 3, 9
     print(f"member groups graph={graph}", flush=True)
     p = hos.params(Nx=Nx, Ny=Ny, nlevs=nlevs, precision=1, depth=0.9)
     states = [random_state(Nx, Ny, nlevs, amp=0.06, seed=40 + i) for i in range(nm)]
-    dts = np.array([0.02, 0.03, 0.025, 0.021, 0.027])
+    dts = 0.02 + 0.001 * np.arange(nm)
     out = {}
-    for ng in (1, 2):
+    for ng in (1, 2, 3, 4):
         s = hos.Solver(p, nm)
         s.set_tile(ng, hos.HOS_TILE_GROUPS)
         assert s.get_tile(hos.HOS_TILE_GROUPS) == ng
@@ -185,15 +185,16 @@ def test_member_groups_match_one_chain(hos, graph):
         for i, u in enumerate(states):
             s.set_state(u, i)
         s.step(3, dts)
-        s.set_active(3)  # one chain below 4 active members
+        s.set_active(5)  # at most 2 groups of two members or more
         s.step(2, dts)
         s.set_active(nm)
         s.step(3, dts)
         out[ng] = [(s.get_state(i), s.energy(i)) for i in range(nm)]
         s.close()
-    for (u1, e1), (u2, e2) in zip(out[1], out[2]):
-        assert np.array_equal(u1, u2)
-        assert e1 == e2
+    for ng in (2, 3, 4):
+        for (u1, e1), (u2, e2) in zip(out[1], out[ng]):
+            assert np.array_equal(u1, u2)
+            assert e1 == e2
 
 
 def test_slots_active_count_and_member_params(hos):
