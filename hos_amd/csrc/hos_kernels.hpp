@@ -708,15 +708,67 @@ struct C2rTile {
     }
 };
 
-// phase B of the RHS pass: N_eta, N_phi at the tasks' points from the physical
-// fields (eta and d^j phi^(a), a + j <= M) and the deferred values, then the y
-// r2c and the interleaved store
+// the parts of the RHS sums that need W^(1) .. W^(M-1) only, from eta and the
+// physical fields d^j phi^(a), a + j <= M (no deferred field): the sum of
+// W^(M-1) .. W^(2) (W^(M) is added last, after phase A), W^(M-2) and
+// (W^2)^(K) for K = M, M-2
+template <class F, bool C>
+struct RhsPre {
+    KSum<F, C> nl;
+    F m2, w2, w22;
+};
+
+// the physical fields d^j phi^(a), a + j <= M, at one task's points (d_pos order)
+template <class T, int M>
+struct RhsPhys {
+    static constexpr int NF = (M - 1) * M / 2;
+    vec_t<T> d[NF > 0 ? NF : 1];
+    __device__ __forceinline__ void load(const Phys<T> &ph, const HosIdx &ix, int m, int x, int y)
+    {
+#pragma unroll
+        for (int a = 1; a < M; ++a)
+#pragma unroll
+            for (int j = 1; a + j <= M; ++j) d[d_pos<M - 1>(a, j)] = ph.vat(m, ix.d[a][j], x, y);
+    }
+};
+
+template <class T, int M, bool C>
+__device__ __forceinline__ RhsPre<vec_t<T>, C> rhs_pre(const HosIdx &ix, const RhsPhys<T, M> &ps, vec_t<T> e)
+{
+    using F = vec_t<T>;
+    const F *d = ps.d;
+    F W[M + 1];
+#pragma unroll
+    for (int a = 1; a < M; ++a) {
+        F w = F(1);
+        KSum<F, C> sum;
+#pragma unroll
+        for (int k = 0; k < a; ++k) {
+            sum.add(w * T(ix.invfact[k]) * d[d_pos<M - 1>(a - k, k + 1)]);
+            w *= e;
+        }
+        W[a] = sum.value();
+    }
+    RhsPre<F, C> r;
+#pragma unroll
+    for (int a = M - 1; a >= 2; --a) r.nl.add(W[a]);
+    r.m2 = F(0);
+    r.w22 = F(0);
+    if constexpr (M - 2 >= 1) {
+        r.m2 = w_upto_v<F, 1, M - 2, C>(W);
+        r.w22 = w2_upto_v<F, M - 2, C>(W);
+    }
+    r.w2 = w2_upto_v<F, M, C>(W);
+    return r;
+}
+
+// phase B of the RHS pass: N_eta, N_phi at the tasks' points from the sums of
+// rhs_pre and the deferred values, then the y r2c and the interleaved store
 template <class T, int LOGNY, int TXP, int M, bool C, int NTT, int PER>
-__device__ __forceinline__ void rhs_phase_b(const Phys<T> &ph, const HosIdx &ix, cx<T> *__restrict__ out, int m,
-                                            int x0, int nx, cx<T> *buf, const cx<T> *__restrict__ twy,
-                                            const int *tp, const int *ty, const vec_t<T> *eta, const vec_t<T> *ex,
-                                            const vec_t<T> *ey, const vec_t<T> *px, const vec_t<T> *py,
-                                            const KSum<vec_t<T>, C> *wm)
+__device__ __forceinline__ void rhs_phase_b(cx<T> *__restrict__ out, int x0, int nx, cx<T> *buf,
+                                            const cx<T> *__restrict__ twy, const int *tp, const int *ty,
+                                            const RhsPre<vec_t<T>, C> *pre, const vec_t<T> *ex, const vec_t<T> *ey,
+                                            const vec_t<T> *px, const vec_t<T> *py, const KSum<vec_t<T>, C> *wm)
 {
     using F = vec_t<T>;
     constexpr int V = Vec<T>::N;
@@ -727,35 +779,13 @@ __device__ __forceinline__ void rhs_phase_b(const Phys<T> &ph, const HosIdx &ix,
     for (int q = 0; q < PER; ++q) {
         const int e = threadIdx.x + q * NTT;
         if ((NTASK % NTT) != 0 && e >= NTASK) continue;
-        const int x = x0 + tp[q], y = ty[q];
-        constexpr int NF = (M - 1) * M / 2;  // d^j phi^(a) with a + j <= M
-        F d[NF > 0 ? NF : 1];
-#pragma unroll
-        for (int a = 1; a < M; ++a)
-#pragma unroll
-            for (int j = 1; a + j <= M; ++j) d[d_pos<M - 1>(a, j)] = ph.vat(m, ix.d[a][j], x, y);
-        F W[M + 1];
-#pragma unroll
-        for (int a = 1; a < M; ++a) {
-            F w = F(1);
-            KSum<F, C> sum;
-#pragma unroll
-            for (int k = 0; k < a; ++k) {
-                sum.add(w * T(ix.invfact[k]) * d[d_pos<M - 1>(a - k, k + 1)]);
-                w *= eta[q];
-            }
-            W[a] = sum.value();
-        }
-        W[M] = wm[q].value();
+        // sum_{a=2}^{M} W^(a): W^(M) joins the compensated sum of W^(M-1) .. W^(2)
+        KSum<F, C> nl = pre[q].nl;
+        if constexpr (M >= 2) nl.add(wm[q].value());
         const F G = ex[q] * ex[q] + ey[q] * ey[q];
-        F wM2 = F(0), w2M2 = F(0);
-        if constexpr (M - 2 >= 1) wM2 = w_upto_v<F, 1, M - 2, C>(W);
-        const F wMnl = w_upto_v<F, 2, M, C>(W);
-        const F na = -(ex[q] * px[q] + ey[q] * py[q]) + wMnl + G * wM2;
-        const F w2M = w2_upto_v<F, M, C>(W);
-        if constexpr (M - 2 >= 1) w2M2 = w2_upto_v<F, M - 2, C>(W);
-        const F nb = T(0.5) * (w2M + G * w2M2) - T(0.5) * (px[q] * px[q] + py[q] * py[q]);
-        cx<T> *o = buf + L::at(tp[q], y);
+        const F na = -(ex[q] * px[q] + ey[q] * py[q]) + nl.value() + G * pre[q].m2;
+        const F nb = T(0.5) * (pre[q].w2 + G * pre[q].w22) - T(0.5) * (px[q] * px[q] + py[q] * py[q]);
+        cx<T> *o = buf + L::at(tp[q], ty[q]);
 #pragma unroll
         for (int v = 0; v < V; ++v) o[v] = {na[v], nb[v]};
     }
@@ -772,16 +802,26 @@ __device__ __forceinline__ void rhs_phase_b(const Phys<T> &ph, const HosIdx &ix,
 }
 
 
+// the physical fields are read before phase A (with the first deferred tiles,
+// one memory latency less per tile) where they are few: FP32 form, M <= 5;
+// their sums then wait in LDS (4 vectors per task, 5 compensated)
+template <class T, int M>
+constexpr bool rhs_early()
+{
+    return sizeof(T) == 4 && (M - 1) * M / 2 <= 10;
+}
+
 template <class T, int LOGNY, int TXP, int M, bool C, int NTT>
 __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, const cx<T> *__restrict__ Im,
                                          long fsI, const RhsDeferred<M> &dfr, cx<T> *__restrict__ out, int m,
-                                         int x0, int nx, cx<T> *buf, const cx<T> *__restrict__ twy)
+                                         int x0, int nx, cx<T> *buf, vec_t<T> *pbuf, const cx<T> *__restrict__ twy)
 {
     using F = vec_t<T>;
     constexpr int V = Vec<T>::N;
     constexpr int NY = 1 << LOGNY, NYV = NY / V, NPR = TXP / 2;
     constexpr int NTASK = TXP * NYV, PER = (NTASK + NTT - 1) / NTT;
     constexpr int ND = M + 4;
+    constexpr bool EARLY = rhs_early<T, M>();
     using L = Lds<T, NY>;
     // this thread's tasks: V consecutive y of x-row p (clamped past the end)
     int tp[PER], ty[PER];
@@ -793,16 +833,41 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
     }
     F eta[PER], ex[PER], ey[PER], px[PER], py[PER];
     KSum<F, C> wm[PER];
-#pragma unroll
-    for (int q = 0; q < PER; ++q) eta[q] = ph.vat(m, ix.eta, x0 + tp[q], ty[q]);
-    // phase A: y c2r of the deferred fields, BF at a time (BF*NPR rows share
-    // the stages and barriers of one LDS FFT); the next batch's tiles are
-    // loaded into the same registers during the current FFT
+    RhsPre<F, C> pre[PER];
     constexpr int BF = 3, NB = (ND + BF - 1) / BF;
     C2rTile<T, LOGNY, NPR, NTT> tile[BF];
 #pragma unroll
-    for (int i = 0; i < BF; ++i)
-        if (i < ND) tile[i].load(Im + dfr.slot[i] * fsI, x0, nx);
+    for (int q = 0; q < PER; ++q) eta[q] = ph.vat(m, ix.eta, x0 + tp[q], ty[q]);
+    if constexpr (EARLY) {
+        // the physical fields first, the first batch of deferred tiles right
+        // after them (loads complete in order: the sums wait for the physical
+        // fields only, the tiles are still in flight)
+        RhsPhys<T, M> ps[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) ps[q].load(ph, ix, m, x0 + tp[q], ty[q]);
+#pragma unroll
+        for (int i = 0; i < BF; ++i)
+            if (i < ND) tile[i].load(Im + dfr.slot[i] * fsI, x0, nx);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = threadIdx.x + q * NTT;
+            if ((NTASK % NTT) != 0 && e >= NTASK) continue;
+            const RhsPre<F, C> r = rhs_pre<T, M, C>(ix, ps[q], eta[q]);
+            F *o = pbuf + e;  // own slots only: no barrier
+            o[0] = r.nl.s;
+            o[NTASK] = r.m2;
+            o[2 * NTASK] = r.w2;
+            o[3 * NTASK] = r.w22;
+            if constexpr (C) o[4 * NTASK] = r.nl.c;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < BF; ++i)
+            if (i < ND) tile[i].load(Im + dfr.slot[i] * fsI, x0, nx);
+    }
+    // phase A: y c2r of the deferred fields, BF at a time (BF*NPR rows share
+    // the stages and barriers of one LDS FFT); the next batch's tiles are
+    // loaded into the same registers during the current FFT
     for (int b = 0; b < NB; ++b) {
         const int nf = min(BF, ND - b * BF);
 #pragma unroll
@@ -845,7 +910,27 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
         }
         __syncthreads();  // buf is refilled next
     }
-    rhs_phase_b<T, LOGNY, TXP, M, C, NTT, PER>(ph, ix, out, m, x0, nx, buf, twy, tp, ty, eta, ex, ey, px, py, wm);
+    if constexpr (EARLY) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = threadIdx.x + q * NTT;
+            if ((NTASK % NTT) != 0 && e >= NTASK) continue;
+            const F *o = pbuf + e;
+            pre[q].nl.s = o[0];
+            pre[q].m2 = o[NTASK];
+            pre[q].w2 = o[2 * NTASK];
+            pre[q].w22 = o[3 * NTASK];
+            if constexpr (C) pre[q].nl.c = o[4 * NTASK];
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            RhsPhys<T, M> ps;
+            ps.load(ph, ix, m, x0 + tp[q], ty[q]);
+            pre[q] = rhs_pre<T, M, C>(ix, ps, eta[q]);
+        }
+    }
+    rhs_phase_b<T, LOGNY, TXP, M, C, NTT, PER>(out, x0, nx, buf, twy, tp, ty, pre, ex, ey, px, py, wm);
 }
 
 // 512 threads: half the per-thread points (and accumulators) of a 256-thread
@@ -861,14 +946,17 @@ __global__ __launch_bounds__(NT_RHS) void k_yr2c_rhs(Phys<T> ph, HosIdx ix, int 
     constexpr int NY = 1 << LOGNY;
     // the r2c of TXP rows, or a batch of 3 deferred fields of TXP/2 rows each
     __shared__ cx<T> buf[Lds<T, NY>::size(TXP > 3 * (TXP / 2) ? TXP : 3 * (TXP / 2))];
+    // the early sums (rhs_early): up to 5 vectors per task
+    constexpr int NTASK = TXP * (NY / Vec<T>::N);
+    __shared__ vec_t<T> pbuf[rhs_early<T, M>() ? 5 * NTASK : 1];
     const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TXP;
     const int m = blockIdx.z;
     const cx<T> *Im = I + m * sI;
     cx<T> *out = I + m * sI + dst * fsI;
     if (comp)
-        rhs_tile<T, LOGNY, TXP, M, true, NT_RHS>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, twy);
+        rhs_tile<T, LOGNY, TXP, M, true, NT_RHS>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, pbuf, twy);
     else
-        rhs_tile<T, LOGNY, TXP, M, false, NT_RHS>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, twy);
+        rhs_tile<T, LOGNY, TXP, M, false, NT_RHS>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, pbuf, twy);
 }
 
 // ---------------------------------------------------------------------------
