@@ -43,14 +43,14 @@ struct Lds {
     __device__ static __forceinline__ int at(int row, int i) { return row * PITCH + i + (i >> 4); }
 };
 
-// Stage plan for NROWS rows of length 2^logn: radix 16, 8, 4 or 2 per stage,
-// the largest radix that keeps at least min(128, NROWS*N/4) butterflies per
-// stage (so most of a 256-thread workgroup stays busy) and does not leave a
-// single radix-2 bit behind.  Fewer stages = fewer LDS round trips.
-__host__ __device__ constexpr int fft_stage_bits(int logn, int nrows, int s)
+// Stage plan for NROWS rows of length 2^logn on nt threads: radix 16, 8, 4 or
+// 2 per stage, the largest radix that keeps at least min(nt/2, NROWS*N/4)
+// butterflies per stage (so most of the workgroup stays busy) and does not
+// leave a single radix-2 bit behind.  Fewer stages = fewer LDS round trips.
+__host__ __device__ constexpr int fft_stage_bits(int logn, int nrows, int s, int nt = 256)
 {
     const long total = (long)nrows << logn;
-    const long thr = total / 4 < 128 ? total / 4 : 128;
+    const long thr = total / 4 < nt / 2 ? total / 4 : nt / 2;
     int rem = logn, bits = 0;
     for (int i = 0; i <= s; ++i) {
         int b = rem == 1 ? 1 : 2;
@@ -65,10 +65,10 @@ __host__ __device__ constexpr int fft_stage_bits(int logn, int nrows, int s)
     }
     return bits;
 }
-__host__ __device__ constexpr int fft_nstages(int logn, int nrows)
+__host__ __device__ constexpr int fft_nstages(int logn, int nrows, int nt = 256)
 {
     int n = 0;
-    while (fft_stage_bits(logn, nrows, n) > 0) ++n;
+    while (fft_stage_bits(logn, nrows, n, nt) > 0) ++n;
     return n;
 }
 
@@ -195,7 +195,7 @@ __device__ __forceinline__ void stage_twiddles(cx<T> *v, int j, const cx<T> *__r
 }
 
 template <class T, int R, int LOGN, int NROWS, int NT, bool INV, int NS>
-__device__ __forceinline__ void fft_stage(cx<T> *buf, const cx<T> *__restrict__ tw)
+__device__ __forceinline__ void fft_stage(cx<T> *buf, const cx<T> *__restrict__ tw, int tid)
 {
     constexpr int N = 1 << LOGN;
     constexpr int NR = N / R;
@@ -205,7 +205,7 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, const cx<T> *__restrict__ 
     cx<T> v[Q][R];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        const int bf = threadIdx.x + q * NT;
+        const int bf = tid + q * NT;
         if ((NBF % NT) == 0 || bf < NBF) {
             const int row = bf / NR, j = bf % NR;
             // L::at(row, j + r NR) as a per-thread base plus a constant per r:
@@ -221,7 +221,7 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, const cx<T> *__restrict__ 
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        const int bf = threadIdx.x + q * NT;
+        const int bf = tid + q * NT;
         if ((NBF % NT) == 0 || bf < NBF) {
             const int row = bf / NR, j = bf % NR;
             const int idxD = (j / NS) * NS * R + (j % NS);
@@ -236,23 +236,36 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, const cx<T> *__restrict__ 
 }
 
 template <class T, int LOGN, int NROWS, int NT, bool INV, int S = 0, int NS = 1>
-__device__ __forceinline__ void fft_stages(cx<T> *buf, const cx<T> *__restrict__ tw)
+__device__ __forceinline__ void fft_stages(cx<T> *buf, const cx<T> *__restrict__ tw, int tid)
 {
-    if constexpr (S < fft_nstages(LOGN, NROWS)) {
-        constexpr int R = 1 << fft_stage_bits(LOGN, NROWS, S);
-        fft_stage<T, R, LOGN, NROWS, NT, INV, NS>(buf, tw);
-        fft_stages<T, LOGN, NROWS, NT, INV, S + 1, NS * R>(buf, tw);
+    if constexpr (S < fft_nstages(LOGN, NROWS, NT)) {
+        constexpr int R = 1 << fft_stage_bits(LOGN, NROWS, S, NT);
+        fft_stage<T, R, LOGN, NROWS, NT, INV, NS>(buf, tw, tid);
+        fft_stages<T, LOGN, NROWS, NT, INV, S + 1, NS * R>(buf, tw, tid);
     }
+}
+
+// threadIdx.x through an opaque copy: what a caller derives from it (LDS and
+// global addresses) is recomputed at each use, a few integer operations,
+// instead of being shared across the kernel and held in registers throughout
+__device__ __forceinline__ int opaque_tid()
+{
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    return tid;
 }
 
 // In-place FFT of NROWS rows of length 2^LOGN held in LDS in the Lds<T, N>
 // layout.  Starts and ends with a workgroup barrier, so callers may write buf
-// just before and read it just after.
+// just before and read it just after.  tw: e^{-2 pi i m/N}; every plan reads
+// m < N/2 only (stage twiddle indices stay below N/R_last, or 4 N/R_last for a
+// last radix of 8 or 16)
 template <class T, int LOGN, int NROWS, int NT, bool INV>
 __device__ __forceinline__ void lds_fft(cx<T> *buf, const cx<T> *__restrict__ tw)
 {
+    const int tid = opaque_tid();  // stage addresses recomputed per call
     __syncthreads();
-    fft_stages<T, LOGN, NROWS, NT, INV>(buf, tw);
+    fft_stages<T, LOGN, NROWS, NT, INV>(buf, tw, tid);
 }
 
 // ---------------------------------------------------------------------------

@@ -708,6 +708,55 @@ struct C2rTile {
     }
 };
 
+// y c2r tile loads of NF fields at once (x-rows x0 .. x0+2 NPR-1 of each):
+// the NF C2rTile index ranges laid end to end over the workgroup, so a
+// thread holds ceil(NF NL / NTT) pieces rather than NF ceil(NL / NTT) (with
+// 512 threads and NL = 528, 4 rather than 6 for three fields)
+template <class T, int LOGNY, int NPR, int NTT, int NF>
+struct C2rBatch {
+    static constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1;
+    static constexpr int NL = ((NK + 15) / 16) * 16 * NPR, NLB = NF * NL, PERL = (NLB + NTT - 1) / NTT;
+    cx<T> A[PERL], B[PERL];
+    __device__ __forceinline__ void load(const cx<T> *__restrict__ base, long fs, const int *slot, int x0, int nx)
+    {
+        const int tid = opaque_tid();
+#pragma unroll
+        for (int q = 0; q < PERL; ++q) {
+            const int e = min(tid + q * NTT, NLB - 1);  // past the end: reload, unused
+            const int i = e / NL, el = e % NL;
+            const int p = (el / 16) % NPR;
+            const int ky = min((el / (16 * NPR)) * 16 + el % 16, NK - 1);
+            int sl = slot[0];  // slot[i] by selects (no indexed private array)
+#pragma unroll
+            for (int k = 1; k < NF; ++k)
+                if (i == k) sl = slot[k];
+            load2(base + sl * fs + (long)ky * nx + x0 + 2 * p, A[q], B[q]);
+        }
+    }
+    // Hermitian fill: field i of the batch to rows i NPR .. i NPR + NPR-1 of buf
+    __device__ __forceinline__ void fill(cx<T> *buf) const
+    {
+        using L = Lds<T, NY>;
+        const int tid = opaque_tid();
+#pragma unroll
+        for (int q = 0; q < PERL; ++q) {
+            const int e = tid + q * NTT;
+            const int i = e / NL, el = e % NL;
+            const int p = (el / 16) % NPR, ky = (el / (16 * NPR)) * 16 + el % 16;
+            if (((NLB % NTT) == 0 || e < NLB) && ky < NK) {
+                cx<T> a = A[q], b = B[q];
+                if (ky == 0 || ky == NY / 2) {
+                    a.y = T(0);
+                    b.y = T(0);
+                }
+                cx<T> *row = buf + (i * NPR + p) * L::PITCH;
+                row[ky + (ky >> 4)] = {a.x - b.y, a.y + b.x};
+                if (ky > 0 && ky < NY / 2) row[L::at(0, NY - ky)] = {a.x + b.y, b.x - a.y};
+            }
+        }
+    }
+};
+
 // the parts of the RHS sums that need W^(1) .. W^(M-1) only, from eta and the
 // physical fields d^j phi^(a), a + j <= M (no deferred field): the sum of
 // W^(M-1) .. W^(2) (W^(M) is added last, after phase A), W^(M-2) and
@@ -814,7 +863,8 @@ constexpr bool rhs_early()
 template <class T, int LOGNY, int TXP, int M, bool C, int NTT>
 __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, const cx<T> *__restrict__ Im,
                                          long fsI, const RhsDeferred<M> &dfr, cx<T> *__restrict__ out, int m,
-                                         int x0, int nx, cx<T> *buf, vec_t<T> *pbuf, const cx<T> *__restrict__ twy)
+                                         int x0, int nx, cx<T> *buf, vec_t<T> *pbuf, cx<T> *twl,
+                                         const cx<T> *__restrict__ twy)
 {
     using F = vec_t<T>;
     constexpr int V = Vec<T>::N;
@@ -834,8 +884,18 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
     F eta[PER], ex[PER], ey[PER], px[PER], py[PER];
     KSum<F, C> wm[PER];
     RhsPre<F, C> pre[PER];
-    constexpr int BF = 3, NB = (ND + BF - 1) / BF;
-    C2rTile<T, LOGNY, NPR, NTT> tile[BF];
+    // deferred fields in batches of BF, the last one of NFL
+    constexpr int BF = 3, NB = (ND + BF - 1) / BF, NFL = ND - (NB - 1) * BF;
+    C2rBatch<T, LOGNY, NPR, NTT, BF> tb;   // batches 0 .. NB-2
+    C2rBatch<T, LOGNY, NPR, NTT, NFL> tl;  // batch NB-1
+    // the twiddles (the first NY/2 of the table: every plan's indices) go to
+    // LDS: read from global memory between the tile prefetch and its use,
+    // they would wait for the prefetch (loads complete in order).  Read
+    // first, stored once the loads after them are waited for.
+    constexpr int NTW = NY / 2, PTW = (NTW + NTT - 1) / NTT;
+    cx<T> twv[PTW];
+#pragma unroll
+    for (int q = 0; q < PTW; ++q) twv[q] = twy[min((int)threadIdx.x + q * NTT, NTW - 1)];
 #pragma unroll
     for (int q = 0; q < PER; ++q) eta[q] = ph.vat(m, ix.eta, x0 + tp[q], ty[q]);
     if constexpr (EARLY) {
@@ -845,9 +905,10 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
         RhsPhys<T, M> ps[PER];
 #pragma unroll
         for (int q = 0; q < PER; ++q) ps[q].load(ph, ix, m, x0 + tp[q], ty[q]);
-#pragma unroll
-        for (int i = 0; i < BF; ++i)
-            if (i < ND) tile[i].load(Im + dfr.slot[i] * fsI, x0, nx);
+        if (NB > 1)
+            tb.load(Im, fsI, dfr.slot, x0, nx);
+        else
+            tl.load(Im, fsI, dfr.slot, x0, nx);
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int e = threadIdx.x + q * NTT;
@@ -861,31 +922,36 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
             if constexpr (C) o[4 * NTASK] = r.nl.c;
         }
     } else {
-#pragma unroll
-        for (int i = 0; i < BF; ++i)
-            if (i < ND) tile[i].load(Im + dfr.slot[i] * fsI, x0, nx);
+        if (NB > 1)
+            tb.load(Im, fsI, dfr.slot, x0, nx);
+        else
+            tl.load(Im, fsI, dfr.slot, x0, nx);
     }
+#pragma unroll
+    for (int q = 0; q < PTW; ++q)
+        if ((NTW % NTT) == 0 || (int)threadIdx.x + q * NTT < NTW) twl[threadIdx.x + q * NTT] = twv[q];
     // phase A: y c2r of the deferred fields, BF at a time (BF*NPR rows share
     // the stages and barriers of one LDS FFT); the next batch's tiles are
-    // loaded into the same registers during the current FFT
+    // loaded into the same registers during the current FFT.  Unrolled, so
+    // each batch's row count is a constant and no tile register is copied
+    // (a copy would wait for the loads in flight)
+#pragma unroll
     for (int b = 0; b < NB; ++b) {
-        const int nf = min(BF, ND - b * BF);
-#pragma unroll
-        for (int i = 0; i < BF; ++i)
-            if (i < nf) tile[i].fill(buf + i * NPR * L::PITCH);
-        if (b + 1 < NB) {
-#pragma unroll
-            for (int i = 0; i < BF; ++i) {
-                const int f = (b + 1) * BF + i;
-                if (f < ND) tile[i].load(Im + dfr.slot[f] * fsI, x0, nx);
-            }
-        }
-        if (nf == 3)
-            lds_fft<T, LOGNY, 3 * NPR, NTT, true>(buf, twy);
-        else if (nf == 2)
-            lds_fft<T, LOGNY, 2 * NPR, NTT, true>(buf, twy);
+        const int nf = b + 1 < NB ? BF : NFL;
+        if (b + 1 < NB)
+            tb.fill(buf);
         else
-            lds_fft<T, LOGNY, NPR, NTT, true>(buf, twy);
+            tl.fill(buf);
+        if (b + 2 < NB)
+            tb.load(Im, fsI, dfr.slot + (b + 1) * BF, x0, nx);
+        else if (b + 1 < NB)
+            tl.load(Im, fsI, dfr.slot + (b + 1) * BF, x0, nx);
+        if (nf == 3)
+            lds_fft<T, LOGNY, 3 * NPR, NTT, true>(buf, twl);
+        else if (nf == 2)
+            lds_fft<T, LOGNY, 2 * NPR, NTT, true>(buf, twl);
+        else
+            lds_fft<T, LOGNY, NPR, NTT, true>(buf, twl);
 #pragma unroll
         for (int i = 0; i < BF; ++i) {
             if (i >= nf) break;
@@ -930,7 +996,7 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
             pre[q] = rhs_pre<T, M, C>(ix, ps, eta[q]);
         }
     }
-    rhs_phase_b<T, LOGNY, TXP, M, C, NTT, PER>(out, x0, nx, buf, twy, tp, ty, pre, ex, ey, px, py, wm);
+    rhs_phase_b<T, LOGNY, TXP, M, C, NTT, PER>(out, x0, nx, buf, twl, tp, ty, pre, ex, ey, px, py, wm);
 }
 
 // 512 threads: half the per-thread points (and accumulators) of a 256-thread
@@ -949,14 +1015,15 @@ __global__ __launch_bounds__(NT_RHS) void k_yr2c_rhs(Phys<T> ph, HosIdx ix, int 
     // the early sums (rhs_early): up to 5 vectors per task
     constexpr int NTASK = TXP * (NY / Vec<T>::N);
     __shared__ vec_t<T> pbuf[rhs_early<T, M>() ? 5 * NTASK : 1];
+    __shared__ cx<T> twl[NY / 2];
     const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TXP;
     const int m = blockIdx.z;
     const cx<T> *Im = I + m * sI;
     cx<T> *out = I + m * sI + dst * fsI;
     if (comp)
-        rhs_tile<T, LOGNY, TXP, M, true, NT_RHS>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, pbuf, twy);
+        rhs_tile<T, LOGNY, TXP, M, true, NT_RHS>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, pbuf, twl, twy);
     else
-        rhs_tile<T, LOGNY, TXP, M, false, NT_RHS>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, pbuf, twy);
+        rhs_tile<T, LOGNY, TXP, M, false, NT_RHS>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, pbuf, twl, twy);
 }
 
 // ---------------------------------------------------------------------------
