@@ -346,10 +346,14 @@ __device__ __forceinline__ void wave_stages(cx<T> *v, cx<T> *row, const cx<T> *_
 
 // unnormalised FFT of the wave's row v[E] (see above); row is the wave's LDS
 // row of Lds<T, N>::PITCH elements
-template <class T, int LOGN, bool INV>
+template <class T, int LOGN, bool INV, bool OPQ = false>
 __device__ __forceinline__ void wave_fft(cx<T> *v, cx<T> *row, const cx<T> *__restrict__ tw, int lane)
 {
     static_assert(wave_nstages(LOGN) > 0, "wave FFT lengths: 512, 1024, 2048");
+    // OPQ: the lane index through an opaque copy, so the addresses are
+    // recomputed per call rather than held in registers between the calls of
+    // a loop (fewer registers, one more wave per SIMD for k_xfwdinv_w)
+    if constexpr (OPQ) asm volatile("" : "+v"(lane));
     wave_stages<T, LOGN, INV>(v, row, tw, lane);
 }
 

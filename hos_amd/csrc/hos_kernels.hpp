@@ -1386,7 +1386,7 @@ __global__ __launch_bounds__(NT) void k_xfwdinv_w(cx<T> *__restrict__ I, long sI
 #pragma unroll
         for (int i = 0; i < E; ++i) kz_th<R>(kxv[i], kyv, h, thd[i], kz[i], th[i]);
     }
-    wave_fft<T, LOGNX, false>(s, row, twx, lane);
+    wave_fft<T, LOGNX, false, true>(s, row, twx, lane);
     R kj[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) kj[i] = R(scale);
@@ -1398,7 +1398,7 @@ __global__ __launch_bounds__(NT) void k_xfwdinv_w(cx<T> *__restrict__ I, long sI
             const T mul = T((j & 1) ? kj[i] * th[i] : kj[i]);  // odd j: tanh(|k| h)
             v[i] = {mul * s[i].x, mul * s[i].y};
         }
-        wave_fft<T, LOGNX, true>(v, row, twx, lane);
+        wave_fft<T, LOGNX, true, true>(v, row, twx, lane);
         cx<T> *out = base + ((j == ndst && dtop >= 0) ? dtop : dst0 + j - 1) * fs + r0;
 #pragma unroll
         for (int i = 0; i < E; ++i) out[lane + 64 * i] = v[i];
