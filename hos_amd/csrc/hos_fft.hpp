@@ -268,6 +268,28 @@ __device__ __forceinline__ void lds_fft(cx<T> *buf, const cx<T> *__restrict__ tw
     fft_stages<T, LOGN, NROWS, NT, INV>(buf, tw, tid);
 }
 
+// The twiddles an lds_fft reads (the first N/2 of the table) staged in LDS.
+// From global memory, the twiddle loads of the stages wait behind whatever
+// the workgroup still has in flight (tile prefetches, stores): loads complete
+// in order.  load() before the tile loads, store() once they are waited for
+// (with the tile's LDS fill); the FFT's opening barrier publishes the copy.
+template <class T, int N, int NTT>
+struct TwLds {
+    static constexpr int NTW = N / 2, P = (NTW + NTT - 1) / NTT;
+    cx<T> v[P];
+    __device__ __forceinline__ void load(const cx<T> *__restrict__ tw)
+    {
+#pragma unroll
+        for (int q = 0; q < P; ++q) v[q] = tw[min((int)threadIdx.x + q * NTT, NTW - 1)];
+    }
+    __device__ __forceinline__ void store(cx<T> *l) const
+    {
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+            if ((NTW % NTT) == 0 || (int)threadIdx.x + q * NTT < NTW) l[threadIdx.x + q * NTT] = v[q];
+    }
+};
+
 // ---------------------------------------------------------------------------
 // Wave FFT: one 64-lane wave transforms one row of N = 64 E complex values
 // held in registers, lane l owning elements l + 64 m (m = 0..E-1), in natural

@@ -252,7 +252,7 @@ struct FieldMap {
 // issued before the first LDS write; one tile per workgroup.
 template <class T, int LOGNY, int NPR>
 __device__ __forceinline__ void c2r_tile(const cx<T> *__restrict__ src, T *__restrict__ dst, int x0, int nx,
-                                         cx<T> *buf, const cx<T> *__restrict__ twy)
+                                         cx<T> *buf, cx<T> *twl, const cx<T> *__restrict__ twy)
 {
     constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1;
     using L = Lds<T, NY>;
@@ -260,6 +260,8 @@ __device__ __forceinline__ void c2r_tile(const cx<T> *__restrict__ src, T *__res
     // side is contiguous while each instruction still reads whole 16-byte
     // pieces of the same ky lines
     constexpr int NL = ((NK + 15) / 16) * 16 * NPR, PERL = (NL + NT - 1) / NT;
+    TwLds<T, NY, NT> tws;  // twiddles read first, staged in LDS with the fill
+    tws.load(twy);
     cx<T> A[PERL], B[PERL];
 #pragma unroll
     for (int q = 0; q < PERL; ++q) {
@@ -283,7 +285,8 @@ __device__ __forceinline__ void c2r_tile(const cx<T> *__restrict__ src, T *__res
             if (ky > 0 && ky < NY / 2) buf[L::at(p, NY - ky)] = {a.x + b.y, b.x - a.y};
         }
     }
-    lds_fft<T, LOGNY, NPR, NT, true>(buf, twy);
+    tws.store(twl);
+    lds_fft<T, LOGNY, NPR, NT, true>(buf, twl);
     // V consecutive y of both rows per thread: 16-byte stores
     using F = vec_t<T>;
     constexpr int V = Vec<T>::N, NYV = NY / V, NS = NPR * NYV, PERS = (NS + NT - 1) / NT;
@@ -312,9 +315,10 @@ __global__ __launch_bounds__(NT) void k_yc2r(const cx<T> *__restrict__ I, long s
 {
     constexpr int NY = 1 << LOGNY, NPR = TX / 2;
     __shared__ cx<T> buf[Lds<T, NY>::size(NPR)];
+    __shared__ cx<T> twl[NY / 2];
     const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TX;
     c2r_tile<T, LOGNY, NPR>(I + blockIdx.z * sI + fm.src[blockIdx.y] * fsI,
-                            P + blockIdx.z * sP + fm.dst[blockIdx.y] * fsP, x0, nx, buf, twy);
+                            P + blockIdx.z * sP + fm.dst[blockIdx.y] * fsP, x0, nx, buf, twl, twy);
 }
 
 // ---------------------------------------------------------------------------
@@ -550,12 +554,14 @@ struct ProZvel {
 // One tile per workgroup.
 template <class T, int LOGNY, int NPR, class PRO>
 __device__ __forceinline__ void r2c_tile(const PRO &pro, int m, cx<T> *__restrict__ out, int x0, int nx,
-                                         cx<T> *buf, const cx<T> *__restrict__ twy)
+                                         cx<T> *buf, cx<T> *twl, const cx<T> *__restrict__ twy)
 {
     using F = vec_t<T>;
     constexpr int V = Vec<T>::N;
     constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1, NYV = NY / V, NTASK = NPR * NYV;
     constexpr int PER = (NTASK + NT - 1) / NT;
+    TwLds<T, NY, NT> tws;  // twiddles read first, staged in LDS with the fill
+    tws.load(twy);
     // one task = V consecutive y of the row pair; the loop is unrolled so the
     // loads of several tasks are in flight together
 #pragma unroll
@@ -570,7 +576,8 @@ __device__ __forceinline__ void r2c_tile(const PRO &pro, int m, cx<T> *__restric
             for (int v = 0; v < V; ++v) d[v] = {ra[v], rb[v]};
         }
     }
-    lds_fft<T, LOGNY, NPR, NT, false>(buf, twy);
+    tws.store(twl);
+    lds_fft<T, LOGNY, NPR, NT, false>(buf, twl);
     // the two real rows of pair p at ky from Z(ky) and Z(NY-ky); 16 consecutive
     // ky of one pair per 16 lanes (contiguous in LDS)
     constexpr int NO = ((NK + 15) / 16) * 16 * NPR;
@@ -592,12 +599,14 @@ __device__ __forceinline__ void r2c_tile(const PRO &pro, int m, cx<T> *__restric
 // write sectors).
 template <class T, int LOGNY, int NROW, class PRO>
 __device__ __forceinline__ void r2c_pair_tile(const PRO &pro, int m, cx<T> *__restrict__ o, int x0, int nx,
-                                              cx<T> *buf, const cx<T> *__restrict__ twy)
+                                              cx<T> *buf, cx<T> *twl, const cx<T> *__restrict__ twy)
 {
     using F = vec_t<T>;
     constexpr int V = Vec<T>::N;
     constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1, NYV = NY / V, NTASK = NROW * NYV;
     constexpr int PER = (NTASK + NT - 1) / NT;
+    TwLds<T, NY, NT> tws;  // twiddles read first, staged in LDS with the fill
+    tws.load(twy);
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int e = threadIdx.x + q * NT;
@@ -610,7 +619,8 @@ __device__ __forceinline__ void r2c_pair_tile(const PRO &pro, int m, cx<T> *__re
             for (int v = 0; v < V; ++v) d[v] = {a[v], b[v]};
         }
     }
-    lds_fft<T, LOGNY, NROW, NT, false>(buf, twy);
+    tws.store(twl);
+    lds_fft<T, LOGNY, NROW, NT, false>(buf, twl);
     constexpr int NO = ((NK + 15) / 16) * 16 * NROW;
     for (int e = threadIdx.x; e < NO; e += NT) {
         const int p = (e / 16) % NROW, ky = (e / (16 * NROW)) * 16 + e % 16;
@@ -629,9 +639,10 @@ __global__ __launch_bounds__(NT) void k_yr2c(PRO pro, cx<T> *__restrict__ I, lon
 {
     constexpr int NY = 1 << LOGNY, NPR = TX / 2;
     __shared__ cx<T> buf[Lds<T, NY>::size(NPR)];
+    __shared__ cx<T> twl[NY / 2];
     const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TX;
     const int m = blockIdx.z;
-    r2c_tile<T, LOGNY, NPR, PRO>(pro, m, I + m * sI + dst * fsI, x0, nx, buf, twy);
+    r2c_tile<T, LOGNY, NPR, PRO>(pro, m, I + m * sI + dst * fsI, x0, nx, buf, twl, twy);
 }
 
 // y r2c pass, two output fields at the same x, written interleaved into the
@@ -642,9 +653,10 @@ __global__ __launch_bounds__(NT) void k_yr2c_pair(PRO pro, cx<T> *__restrict__ I
 {
     constexpr int NY = 1 << LOGNY;
     __shared__ cx<T> buf[Lds<T, NY>::size(TXP)];
+    __shared__ cx<T> twl[NY / 2];
     const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TXP;
     const int m = blockIdx.z;
-    r2c_pair_tile<T, LOGNY, TXP, PRO>(pro, m, I + m * sI + dst * fsI, x0, nx, buf, twy);
+    r2c_pair_tile<T, LOGNY, TXP, PRO>(pro, m, I + m * sI + dst * fsI, x0, nx, buf, twl, twy);
 }
 
 // ---------------------------------------------------------------------------
@@ -888,14 +900,9 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
     constexpr int BF = 3, NB = (ND + BF - 1) / BF, NFL = ND - (NB - 1) * BF;
     C2rBatch<T, LOGNY, NPR, NTT, BF> tb;   // batches 0 .. NB-2
     C2rBatch<T, LOGNY, NPR, NTT, NFL> tl;  // batch NB-1
-    // the twiddles (the first NY/2 of the table: every plan's indices) go to
-    // LDS: read from global memory between the tile prefetch and its use,
-    // they would wait for the prefetch (loads complete in order).  Read
-    // first, stored once the loads after them are waited for.
-    constexpr int NTW = NY / 2, PTW = (NTW + NTT - 1) / NTT;
-    cx<T> twv[PTW];
-#pragma unroll
-    for (int q = 0; q < PTW; ++q) twv[q] = twy[min((int)threadIdx.x + q * NTT, NTW - 1)];
+    // the twiddles to LDS (TwLds): the FFTs run while the next batch loads
+    TwLds<T, NY, NTT> tws;
+    tws.load(twy);
 #pragma unroll
     for (int q = 0; q < PER; ++q) eta[q] = ph.vat(m, ix.eta, x0 + tp[q], ty[q]);
     if constexpr (EARLY) {
@@ -927,9 +934,7 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
         else
             tl.load(Im, fsI, dfr.slot, x0, nx);
     }
-#pragma unroll
-    for (int q = 0; q < PTW; ++q)
-        if ((NTW % NTT) == 0 || (int)threadIdx.x + q * NTT < NTW) twl[threadIdx.x + q * NTT] = twv[q];
+    tws.store(twl);
     // phase A: y c2r of the deferred fields, BF at a time (BF*NPR rows share
     // the stages and barriers of one LDS FFT); the next batch's tiles are
     // loaded into the same registers during the current FFT.  Unrolled, so
