@@ -328,8 +328,11 @@ struct Eng {
         const double scale = 1.0 / ((double)c->p.Nx * c->p.Ny);
         // FP64 [2][Nk][Nx] fields moved by the epilogue
         double f64 = 0.0;
-        if (rk.mode == EPI_RK_SUBDIAG)
-            f64 = 2 /*Y*/ + 2 /*u*/ + (rk.stage > 0 ? 2 : 0) /*acc*/ + 2 /*acc or u*/ + 2 /*Y next*/;
+        if (rk.mode == EPI_RK_SUBDIAG) {
+            // stage 0 reads the state as its input; the last writes the state only
+            const bool first = rk.stage == 0, last = rk.stage == rk.nstages - 1;
+            f64 = 2 /*Y*/ + (first ? 0 : 4) /*u, acc*/ + 2 /*acc or u*/ + (last ? 0 : 2) /*Y next*/;
+        }
         else if (rk.mode == EPI_STORE_K)
             f64 = 2 + 2 + (rk.hpart && rk.stage == 0 ? 2 : 0);
         else

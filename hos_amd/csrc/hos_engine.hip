@@ -448,7 +448,8 @@ static int launch_step(hos_ctx *c, int m0, int n, hipStream_t st)
         if (c->subdiag) {
             rk.mode = EPI_RK_SUBDIAG;
             rk.b_next = (s + 1 < c->S) ? c->b[s + 1][s] : 0.0;
-            rc = e.rhs(c->Y + m0 * s2, rk);
+            // stage 0 reads the state itself, later stages the stage input Y
+            rc = e.rhs((s == 0 ? c->U : c->Y) + m0 * s2, rk);
         } else {
             // generic Butcher (one chain, m0 = 0): K_s stored, then
             // Y_{s+1} = u + dt sum_p b[s+1][p] K_p

@@ -47,12 +47,17 @@ struct Lds {
 // 2 per stage, the largest radix that keeps at least min(nt/2, NROWS*N/4)
 // butterflies per stage (so most of the workgroup stays busy) and does not
 // leave a single radix-2 bit behind.  Fewer stages = fewer LDS round trips.
+// Above 256 threads the bound is min(nt/2, NROWS*N/8), and the first stage,
+// which has no twiddles, needs half of it (512 threads, one to three rows of
+// 1024: 16 8 8, measured ahead of 8 8 4 4).
 __host__ __device__ constexpr int fft_stage_bits(int logn, int nrows, int s, int nt = 256)
 {
     const long total = (long)nrows << logn;
-    const long thr = total / 4 < nt / 2 ? total / 4 : nt / 2;
+    const long part = nt > 256 ? total / 8 : total / 4;
+    const long thr1 = part < nt / 2 ? part : nt / 2;
     int rem = logn, bits = 0;
     for (int i = 0; i <= s; ++i) {
+        const long thr = (i == 0 && nt > 256) ? thr1 / 2 : thr1;
         int b = rem == 1 ? 1 : 2;
         if (4 <= rem && rem - 4 != 1 && (total >> 4) >= thr)
             b = 4;

@@ -1172,11 +1172,13 @@ __device__ __forceinline__ void rk_epilogue(const RkArgs &rk, const KTab &kt, in
         }
         return;
     }
-    // EPI_RK_SUBDIAG: Y_{s+1} = u + dt b[s+1][s] K_s ; acc += dt c[s] K_s
+    // EPI_RK_SUBDIAG: Y_{s+1} = u + dt b[s+1][s] K_s ; acc += dt c[s] K_s.
+    // Stage 0's input is the state itself (y == u, not read twice); the last
+    // stage writes the state only, which the next step's stage 0 reads.
     cx<double> *u = rk.U + m * 2 * fsd;
     cx<double> *acc = rk.A + m * 2 * fsd;
     cx<double> *yn = rk.Yn + m * 2 * fsd;
-    const cx<double> ue = u[idx], up = u[fsd + idx];
+    const cx<double> ue = rk.stage == 0 ? ye : u[idx], up = rk.stage == 0 ? yp : u[fsd + idx];
     if (rk.stage == 0 && rk.hpart) {
         // Hamiltonian, hosm_2d_par.c:462-523: weight 1 on row j=0, 2 elsewhere
         const double w = (ky == 0) ? 1.0 : 2.0;
@@ -1198,8 +1200,6 @@ __device__ __forceinline__ void rk_epilogue(const RkArgs &rk, const KTab &kt, in
         if (rk.mean_level && idx == 0) ae = {rk.mean0[m], 0.0};
         u[idx] = ae;
         u[fsd + idx] = ap;
-        yn[idx] = ae;
-        yn[fsd + idx] = ap;
     } else {
         acc[idx] = ae;
         acc[fsd + idx] = ap;
