@@ -337,7 +337,6 @@ struct Eng {
             f64 = 2 + 2 + (rk.hpart && rk.stage == 0 ? 2 : 0);
         else
             f64 = 2;
-        if (rk.mode != EPI_STORE_SPEC2) f64 += 0.5;  // tanh(|k| h) table
         LOGN_SWITCH(c->lognx, {
             constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
             dim3 grid((c->Nk + ROWS - 1) / ROWS, 1, nm);

@@ -1150,8 +1150,11 @@ __device__ __forceinline__ void rk_epilogue(const RkArgs &rk, const KTab &kt, in
         return;
     }
     const cx<double> ye = y[idx], yp = y[fsd + idx];
-    const double kxv = kt.kx[kx], kyv = ky * kt.ky0;
-    const double w1 = sqrt(kxv * kxv + kyv * kyv) * kt.th[m * kt.ths + idx];  // |k| tanh(|k| h)
+    const double kxv = kt.kx[kx], kyv = ky * kt.ky0, h = kt.h(m);
+    // |k| tanh(|k| h), the expression of k_th_table (same value): computed,
+    // not read, since this pass is bound by its FP64 traffic
+    const double kz = sqrt(kxv * kxv + kyv * kyv);
+    const double w1 = kz * (h > 0.0 ? tanh(kz * h) : 1.0);
     // eta_t = W^(1) + R*(rest), phi_t = -g eta + R*(rest)  (hosm_2d_par.c:211-229)
     cx<double> ke = {w1 * yp.x + R * scale * double(na.x), w1 * yp.y + R * scale * double(na.y)};
     cx<double> kp = {-rk.g * ye.x + R * scale * double(nb.x), -rk.g * ye.y + R * scale * double(nb.y)};
