@@ -195,8 +195,8 @@ def main():
         print(json.dumps(t))
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); members of an ensemble "
-              "shard across GPUs with no communication, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the members of an "
+              "ensemble would shard across GPUs with no communication between them", file=sys.stderr)
         sys.exit(2)
 
     import hos_amd as hos

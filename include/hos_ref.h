@@ -2,8 +2,8 @@
  * hos_ref.h -- the reference solver's own call boundary (Csource/2dpar, the
  * USE_DOUBLES build), implemented by libhos_ref.so on top of the native API in
  * hos_gpu.h.  A driver written against Csource/2dpar (main_2d_par.c) links
- * against libhos_ref.so instead of time_schemes_par.o, hosm_2d_par.o,
- * operators_par.o and fft_routines_2d_par.o; see INTEGRATION.md.
+ * against libhos_ref.so (and libhos_gpu.so, with -fcommon) instead of
+ * time_schemes_par.o, hosm_2d_par.o, operators_par.o and fft_routines_2d_par.o.
  *
  * Types: hos_cplx is double[2], the same type as fftw_complex, so pointers
  * declared fftw_complex* pass unchanged.  hos_plan stands in for fftw_plan; the

@@ -37,7 +37,7 @@ def test_fft_2d_vs_numpy(hos, Nx, Ny, prec):
     assert rel_l2(back, u) < tol
 
 
-@pytest.mark.parametrize("nlevs", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("nlevs", [0, 1, 2, 3, 4, 5, 7])
 @pytest.mark.parametrize("depth", [0.0, 0.7])
 def test_rhs_fp64_vs_oracle(hos, oracle_mod, nlevs, depth):
     Nx, Ny = 64, 32
@@ -99,8 +99,10 @@ def test_xpass_modes_agree(hos, oracle_mod, Nx):
     assert rel_l2(a, b) < 1e-6
 
 
-@pytest.mark.parametrize("nlevs", [1, 2, 4])
+@pytest.mark.parametrize("nlevs", [1, 2, 4, 5, 7])
 def test_rhs_fp32_vs_oracle(hos, oracle_mod, nlevs):
+    """FP32 RHS for M = 2..8: up to M = 5 the RHS pass reads its physical
+    fields before the deferred ones, above that after them."""
     Nx, Ny = 128, 64
     print(f"rhs fp32 nlevs={nlevs}", flush=True)
     o = oracle_mod.Oracle(Nx, Ny, nlevs, 2 * np.pi, np.pi)
