@@ -126,6 +126,13 @@ struct XinvOps {
     int ngroups;
 };
 
+// y passes with one row per wave (wave_fft): the FP32 form at Ny = 512, 1024
+template <class T>
+__host__ __device__ constexpr bool y_wave(int logny)
+{
+    return sizeof(T) == 4 && (logny == 9 || logny == 10);
+}
+
 // XCD-aware tile order: the 8 XCDs take workgroups round-robin, so give
 // consecutive logical tiles to workgroups on the same XCD; tiles that share
 // 128-byte lines of the transposed intermediate then meet in one L2.
@@ -286,6 +293,28 @@ __device__ __forceinline__ void c2r_tile(const cx<T> *__restrict__ src, T *__res
         }
     }
     tws.store(twl);
+    if constexpr (y_wave<T>(LOGNY)) {
+        // one complex row per wave: the wave FFT in registers (no workgroup
+        // barrier between its stages), the two real rows stored from there
+        __syncthreads();  // the rows are filled
+        constexpr int E = NY / 64;
+        const int lane = threadIdx.x & 63;
+        for (int r = threadIdx.x >> 6; r < NPR; r += NT / 64) {
+            cx<T> *row = buf + r * L::PITCH;
+            cx<T> v[E];
+#pragma unroll
+            for (int m = 0; m < E; ++m) v[m] = row[L::at(0, lane + 64 * m)];
+            wave_sync();  // read before the FFT's first exchange writes the row
+            wave_fft<T, LOGNY, true>(v, row, twl, lane);
+            T *d0 = dst + (long)(x0 + 2 * r) * NY, *d1 = d0 + NY;
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                d0[lane + 64 * m] = v[m].x;
+                d1[lane + 64 * m] = v[m].y;
+            }
+        }
+        return;
+    }
     lds_fft<T, LOGNY, NPR, NT, true>(buf, twl);
     // V consecutive y of both rows per thread: 16-byte stores
     using F = vec_t<T>;
