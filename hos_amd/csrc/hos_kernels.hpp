@@ -710,47 +710,9 @@ struct RhsDeferred {
     int slot[M + 4];  // x-intermediate slot of eta_x, eta_y, phi_x, phi_y, then W^(M) term k = 0..M-1
 };
 
-// y c2r tile load: the transposed tile of x-intermediate field src (x-rows
-// x0 .. x0+2 NPR-1) into registers, 16 consecutive ky of one row pair per 16
-// lanes (as c2r_tile)
-template <class T, int LOGNY, int NPR, int NTT = NT>
-struct C2rTile {
-    static constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1;
-    static constexpr int NL = ((NK + 15) / 16) * 16 * NPR, PERL = (NL + NTT - 1) / NTT;
-    cx<T> A[PERL], B[PERL];
-    __device__ __forceinline__ void load(const cx<T> *__restrict__ src, int x0, int nx)
-    {
-#pragma unroll
-        for (int q = 0; q < PERL; ++q) {
-            const int e = threadIdx.x + q * NTT;
-            const int p = (e / 16) % NPR;
-            const int ky = min((e / (16 * NPR)) * 16 + e % 16, NK - 1);  // past the end: reload, unused
-            load2(src + (long)ky * nx + x0 + 2 * p, A[q], B[q]);
-        }
-    }
-    // Hermitian fill of the NPR complex rows in buf (Lds<T, NY> layout)
-    __device__ __forceinline__ void fill(cx<T> *buf) const
-    {
-        using L = Lds<T, NY>;
-#pragma unroll
-        for (int q = 0; q < PERL; ++q) {
-            const int e = threadIdx.x + q * NTT;
-            const int p = (e / 16) % NPR, ky = (e / (16 * NPR)) * 16 + e % 16;
-            if (((NL % NTT) == 0 || e < NL) && ky < NK) {
-                cx<T> a = A[q], b = B[q];
-                if (ky == 0 || ky == NY / 2) {
-                    a.y = T(0);
-                    b.y = T(0);
-                }
-                buf[L::at(p, ky)] = {a.x - b.y, a.y + b.x};
-                if (ky > 0 && ky < NY / 2) buf[L::at(p, NY - ky)] = {a.x + b.y, b.x - a.y};
-            }
-        }
-    }
-};
-
-// y c2r tile loads of NF fields at once (x-rows x0 .. x0+2 NPR-1 of each):
-// the NF C2rTile index ranges laid end to end over the workgroup, so a
+// y c2r tile loads of NF fields at once (x-rows x0 .. x0+2 NPR-1 of each,
+// read transposed as in c2r_tile: 16 consecutive ky of one row pair per 16
+// lanes).  The NF fields' index ranges lie end to end over the workgroup, so a
 // thread holds ceil(NF NL / NTT) pieces rather than NF ceil(NL / NTT) (with
 // 512 threads and NL = 528, 4 rather than 6 for three fields)
 template <class T, int LOGNY, int NPR, int NTT, int NF>
