@@ -693,11 +693,11 @@ __global__ __launch_bounds__(NT) void k_yr2c_pair(PRO pro, cx<T> *__restrict__ I
 // are read by nothing but the RHS prologue: the gradients eta_x, eta_y, phi_x,
 // phi_y and the M terms of W^(M), d^{k+1} phi^(M-k)/dz^{k+1}, k = 0..M-1 (the
 // fields with order + derivative = M+1).  Instead of a y c2r to the physical
-// grid and a read back, this kernel does their y c2r itself, one field after
-// the other in LDS (the next field's tile is loaded during the current FFT),
-// and keeps at each of its points only the four gradients and the compensated
-// sum W^(M).  The other fields (eta and d^j phi^(a) with a+j <= M) come from
-// the physical grid as before.  Then the nonlinear RHS in physical space
+// grid and a read back, this kernel does their y c2r itself, three fields at a
+// time in LDS (the next three tiles load during the current FFT), and keeps at
+// each of its points only the four gradients and the compensated sum W^(M).
+// The other fields (eta and d^j phi^(a) with a+j <= M) come from the physical
+// grid (see rhs_early for when they are read).  Then the nonlinear RHS in physical space
 // (rhs_hos, hosm_2d_par.c:149-206) without the two linear terms W^(1) and
 // -g eta, which the x epilogue adds in FP64:
 //   N_eta = -(eta_x phi_x + eta_y phi_y) + (W^(M) - W^(1)) + |grad eta|^2 W^(M-2)
