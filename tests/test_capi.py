@@ -54,6 +54,31 @@ def test_params_default_and_footprint(hos):
     assert hos.footprint(bad, 1) == 0
 
 
+@pytest.mark.parametrize("kw,nm,msg", [
+    (dict(Nx=1000), 1, "powers of two"),
+    (dict(Ny=8), 1, "powers of two"),
+    (dict(Nx=4096), 1, "powers of two"),
+    (dict(nlevs=8), 1, "nlevs"),
+    (dict(nlevs=-1), 1, "nlevs"),
+    (dict(Lx=0.0), 1, "domain lengths"),
+    (dict(precision=7), 1, "precision"),
+    (dict(rampflg=1, Tramp=0.0), 1, "Tramp"),
+    (dict(), 0, "nmembers"),
+    (dict(scheme=9), 1, "scheme"),
+])
+def test_create_rejects_bad_params(hos, kw, nm, msg):
+    """hos_create checks its arguments before any device work: each bad
+    parameter returns HOS_E_INVALID with a message naming it (no GPU needed)."""
+    base = dict(Nx=64, Ny=64, nlevs=2)
+    base.update(kw)
+    p = hos.params(**base)
+    h = ctypes.c_void_p()
+    rc = hos.load().hos_create(ctypes.byref(p), nm, ctypes.byref(h))
+    assert rc == hos.HOS_E_INVALID
+    assert not h.value
+    assert msg in hos.load().hos_last_error().decode()
+
+
 def test_bytes_per_point_model(hos):
     # B(M,p) = S * (F(M)*4p + 64), F(M) = M(M+1)/2 + M + 6
     p = hos.params(nlevs=4, precision=hos.HOS_FP32)
