@@ -616,6 +616,13 @@ static int step_finish(hos_ctx *c)
     return HOS_OK;
 }
 
+// entry points that write the state, the time or the context's scratch: not
+// while a hos_step_async is in flight (its kernels read them)
+static int idle(const hos_ctx *c)
+{
+    return c->async_pending ? fail(HOS_E_STATE, "a hos_step_async is in flight (hos_wait first)") : HOS_OK;
+}
+
 static int upload_dt(hos_ctx *c, const double *dt)
 {
     for (int m = 0; m < c->nm; ++m) c->host_dt[m] = dt[m];
@@ -626,6 +633,9 @@ static int upload_dt(hos_ctx *c, const double *dt)
 extern "C" int hos_step_dt(hos_ctx *c, int nsteps, const double *dt)
 {
     if (!c || !dt) return fail(HOS_E_INVALID, "null argument");
+    // checked before the dt upload: a step in flight reads the device dt
+    if (int e = idle(c)) return e;
+    if (nsteps < 0) return fail(HOS_E_INVALID, "nsteps < 0");
     int rc = upload_dt(c, dt);
     if (!rc) rc = step_enqueue(c, nsteps);
     if (!rc) rc = step_finish(c);
@@ -775,9 +785,9 @@ extern "C" int hos_set_tile(hos_ctx *c, int kind, int tx)
             return fail(HOS_E_INVALID, "tile %d not available (4, 8 or 16 rows, <= 64 KiB LDS)", v);
         if (c->p.Nx % v) return fail(HOS_E_INVALID, "tile %d does not divide Nx", v);
     }
+    if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
     for (int k = HOS_TILE_C2R; k <= HOS_TILE_PAIR; ++k)
         if (kind == HOS_TILE_ALL || kind == k) c->tx[k] = tx ? tx : default_tx(c->f32, c->logny, k);
-    if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
     drop_graphs(c);
     return HOS_OK;
 }
@@ -924,10 +934,12 @@ static int check_member(hos_ctx *c, int m)
     return HOS_OK;
 }
 
+
 extern "C" int hos_set_state(hos_ctx *c, int member, const double *u)
 {
     int rc = check_member(c, member);
     if (rc) return rc;
+    if (int e = idle(c)) return e;
     if (!u) return fail(HOS_E_INVALID, "null state");
     const size_t n = 2 * c->fs * sizeof(cx<double>);
     HIP_TRY(hipMemcpyAsync(c->U + member * 2 * c->fs, u, n, hipMemcpyHostToDevice, c->stream));
@@ -953,6 +965,7 @@ extern "C" int hos_set_time(hos_ctx *c, int member, double t)
 {
     int rc = check_member(c, member);
     if (rc) return rc;
+    if (int e = idle(c)) return e;
     c->host_t[member] = t;
     HIP_TRY(hipMemcpyAsync(c->t + member, &c->host_t[member], sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1008,6 +1021,7 @@ static int rhs_impl(hos_ctx *c, double t)
 extern "C" int hos_rhs(hos_ctx *c, const double *u, double *hrhs, double t)
 {
     if (!c || !u || !hrhs) return fail(HOS_E_INVALID, "null argument");
+    if (int e = idle(c)) return e;
     const size_t n = 2 * c->fs * sizeof(cx<double>);
     HIP_TRY(hipMemcpyAsync(c->Y, u, n, hipMemcpyHostToDevice, c->stream));
     int rc = c->f32 ? rhs_impl<float>(c, t) : rhs_impl<double>(c, t);
@@ -1037,6 +1051,7 @@ static int zvel_impl(hos_ctx *c)
 extern "C" int hos_zvel(hos_ctx *c, const double *u, double *wM, double *wM2, double *w2M, double *w2M2)
 {
     if (!c || !u) return fail(HOS_E_INVALID, "null argument");
+    if (int e = idle(c)) return e;
     HIP_TRY(hipMemcpyAsync(c->Y, u, 2 * c->fs * sizeof(cx<double>), hipMemcpyHostToDevice, c->stream));
     int rc = c->f32 ? zvel_impl<float>(c) : zvel_impl<double>(c);
     if (rc) return rc;
@@ -1067,6 +1082,7 @@ static int fft_impl(hos_ctx *c)
 extern "C" int hos_fft_2d(hos_ctx *c, const double *u_padded, double *hu)
 {
     if (!c || !u_padded || !hu) return fail(HOS_E_INVALID, "null argument");
+    if (int e = idle(c)) return e;
     const int Nx = c->p.Nx, Ny = c->p.Ny;
     HIP_TRY(hipMemcpy2DAsync(c->hostbuf, sizeof(double) * Ny, u_padded, sizeof(double) * (Ny + 2),
                              sizeof(double) * Ny, Nx, hipMemcpyHostToDevice, c->stream));
@@ -1095,6 +1111,7 @@ static int ifft_impl(hos_ctx *c)
 extern "C" int hos_ifft_2d(hos_ctx *c, const double *hu, double *u_padded)
 {
     if (!c || !u_padded || !hu) return fail(HOS_E_INVALID, "null argument");
+    if (int e = idle(c)) return e;
     const int Nx = c->p.Nx, Ny = c->p.Ny;
     HIP_TRY(hipMemcpyAsync(c->Y, hu, c->fs * sizeof(cx<double>), hipMemcpyHostToDevice, c->stream));
     int rc = c->f32 ? ifft_impl<float>(c) : ifft_impl<double>(c);
@@ -1133,6 +1150,7 @@ static int mult_impl(hos_ctx *c)
 extern "C" int hos_mult(hos_ctx *c, const double *a, const double *b, double *out)
 {
     if (!c || !a || !b || !out) return fail(HOS_E_INVALID, "null argument");
+    if (int e = idle(c)) return e;
     const size_t n = c->fs * sizeof(cx<double>);
     HIP_TRY(hipMemcpyAsync(c->Y, a, n, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->Y + c->fs, b, n, hipMemcpyHostToDevice, c->stream));
@@ -1146,6 +1164,7 @@ extern "C" int hos_mult(hos_ctx *c, const double *a, const double *b, double *ou
 extern "C" int hos_spectral_op(hos_ctx *c, int op, const double *a, const double *b, double *out)
 {
     if (!c || !a || !out) return fail(HOS_E_INVALID, "null argument");
+    if (int e = idle(c)) return e;
     if (op < HOS_OP_DX || op > HOS_OP_DEALIAS) return fail(HOS_E_INVALID, "unknown spectral op %d", op);
     if (op == HOS_OP_SUM && !b) return fail(HOS_E_INVALID, "sum needs two inputs");
     const size_t n = c->fs * sizeof(cx<double>);
@@ -1164,6 +1183,7 @@ extern "C" int hos_hamiltonian(hos_ctx *c, const double *heta, const double *het
                                double *H)
 {
     if (!c || !heta || !heta_t || !hphi || !H) return fail(HOS_E_INVALID, "null argument");
+    if (int e = idle(c)) return e;
     const size_t n = c->fs * sizeof(cx<double>);
     HIP_TRY(hipMemcpyAsync(c->K, heta, n, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->K + c->fs, heta_t, n, hipMemcpyHostToDevice, c->stream));

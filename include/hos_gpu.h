@@ -109,7 +109,13 @@ int hos_step_dt(hos_ctx *c, int nsteps, const double *dt);
  *                         per-member dt already set, and return at once;
  *  hos_wait(c)            wait for them and record the step timings.
  * Contexts use their own HIP stream, so the steps of several contexts
- * enqueued with hos_step_async run concurrently on the device. */
+ * enqueued with hos_step_async run concurrently on the device.  Until
+ * hos_wait, the calls that write the state, the time, the launch shapes or
+ * the context's scratch (hos_step*, hos_set_state, hos_set_time,
+ * hos_set_member, hos_set_active, hos_copy_member, hos_set_tile,
+ * hos_autotune, hos_rhs, hos_zvel, hos_fft_2d, hos_ifft_2d, hos_mult,
+ * hos_spectral_op, hos_hamiltonian) return HOS_E_STATE and change nothing;
+ * the getters wait for the steps in flight. */
 int hos_set_active(hos_ctx *c, int n);
 int hos_get_active(hos_ctx *c);
 int hos_set_member(hos_ctx *c, int member, double depth, double dt, double t);

@@ -150,6 +150,52 @@ def test_steps_vs_oracle(hos, oracle_mod, scheme):
     assert abs(s.get_time() - t) < 1e-12
 
 
+def test_api_misuse_is_an_error_and_leaves_the_state(hos):
+    """Bad member indices, active counts, tiles and step counts return
+    HOS_E_INVALID (HosError) and leave the members' state and time as they were;
+    while a step_async is in flight, calls that write the state, the time, the
+    tiles or the shared scratch return HOS_E_STATE and change nothing."""
+    Nx, Ny, nlevs, nm = 32, 32, 2, 2
+    print("api misuse", flush=True)
+    p = hos.params(Nx=Nx, Ny=Ny, nlevs=nlevs, precision=1)
+    s = hos.Solver(p, nm)
+    u = random_state(Nx, Ny, nlevs, amp=0.05, seed=9)
+    s.set_state(u, 0)
+    s.set_state(u, 1)
+    s.step(2, 0.02)
+    before = [s.get_state(i) for i in range(nm)], [s.get_time(i) for i in range(nm)]
+    bad = [
+        lambda: s.set_state(u, nm),
+        lambda: s.set_state(u, -1),
+        lambda: s.get_state(nm),
+        lambda: s.set_active(nm + 1),
+        lambda: s.set_active(-1),
+        lambda: s.copy_member(0, nm),
+        lambda: s.set_member(nm, 0.0, 0.02),
+        lambda: s.set_tile(5, hos.HOS_TILE_C2R),
+        lambda: s.set_tile(3, hos.HOS_TILE_XPASS),
+        lambda: s.set_tile(5, hos.HOS_TILE_GROUPS),  # 1 .. 4 groups
+        lambda: s.step(-1, 0.02),
+    ]
+    for f in bad:
+        with pytest.raises(hos.HosError):
+            f()
+    after = [s.get_state(i) for i in range(nm)], [s.get_time(i) for i in range(nm)]
+    for a, b in zip(before[0], after[0]):
+        assert np.array_equal(a, b)
+    assert before[1] == after[1]
+    s.step_async(1)
+    for f in (lambda: s.step(1, 0.02), lambda: s.set_state(u, 0), lambda: s.rhs(u),
+              lambda: s.set_tile(8, hos.HOS_TILE_C2R), lambda: s.set_active(1)):
+        with pytest.raises(hos.HosError):
+            f()
+    s.wait()
+    t1 = s.get_time(0)
+    assert t1 == pytest.approx(before[1][0] + 0.02)  # the step in flight kept its dt
+    s.step(1, 0.02)  # usable again
+    s.close()
+
+
 def test_members_match_alone(hos):
     """Each member of a batch gives the result it gives alone (bit-identical)."""
     Nx, Ny, nlevs = 64, 64, 3
