@@ -7,7 +7,7 @@ rounding order), FP32 path <= 2e-5 relative L2 for one RHS.
 import numpy as np
 import pytest
 
-from helpers import random_state, to_spec
+from helpers import dealias_mask, harmonic_state, random_state, to_phys, to_spec, zakharov_rhs
 from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -125,6 +125,39 @@ def test_zvel_vs_oracle(hos, oracle_mod, nlevs):
             assert np.linalg.norm(g) < 1e-14
         else:
             assert rel_l2(g, r) < TOL64
+
+
+@pytest.mark.parametrize("depth", [0.0, 0.8])
+def test_truncation_order_in_amplitude(hos, depth):
+    """The FP64 GPU path on its own (no oracle) against exact harmonic
+    potentials scaled by eps: W^(M), (W^2)^(M) and both RHS components miss the
+    exact values (the Zakharov form with exact W for the RHS) by
+    O(eps^(M+1)), W^(M-2) by O(eps^max(M-1,1)) and (W^2)^(M-2) by
+    O(eps^max(M-1,2)).  The error slopes between eps = 1/4, 1/8 and 1/16 must
+    match those orders: a term of the fused algorithm dropped or mis-weighted
+    at order n leaves a slope of n."""
+    Nx = Ny = 64
+    L = 2 * np.pi
+    g = 9.81
+    for nlevs in range(0, 5):
+        M = nlevs + 1
+        print(f"truncation order depth={depth} M={M}", flush=True)
+        s = make(hos, Nx, Ny, nlevs, 0, depth=depth, g=g)
+        keep = ~dealias_mask(Nx, Ny, nlevs)
+        errs = []
+        for eps in (0.25, 0.125, 0.0625):
+            u, W, eta, phi = harmonic_state(Nx, Ny, L, depth, amp_eta=0.08 * eps, amp_phi=0.3 * eps)
+            wM, wM2, w2M, w2M2 = (to_phys(w, Nx, Ny) for w in s.zvel(u))
+            et, pt = zakharov_rhs(eta, phi, W, L, g)
+            r = s.rhs(u)
+            errs.append([np.abs(wM - W).max(), np.abs(wM2 - W).max(), np.abs(w2M - W * W).max(),
+                         np.abs(w2M2 - W * W).max(), np.abs(r[0] - to_spec(et) * keep).max(),
+                         np.abs(r[1] - to_spec(pt) * keep).max()])
+        s.close()
+        errs = np.array(errs)
+        slopes = np.log2(errs[:-1] / errs[1:])
+        want = np.array([M + 1, max(M - 1, 1), M + 1, max(M - 1, 2), M + 1, M + 1])
+        assert np.all(np.abs(slopes - want) < 0.15), (M, slopes, want)
 
 
 @pytest.mark.parametrize("scheme", [1, 2, 3, 4, 5])

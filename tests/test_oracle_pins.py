@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from conftest import rel_l2
-from helpers import dealias_mask, grid, random_state, to_phys, to_spec
+from helpers import dealias_mask, grid, harmonic_state, random_state, to_phys, to_spec, zakharov_rhs
 
 
 def brute_r2c(u):
@@ -129,29 +129,6 @@ def test_mult_single_modes_with_aliasing(oracle_mod):
 
 
 # ---------------------------------------------------------------- Zvel (A4)
-def harmonic_state(Nx, Ny, L, depth, amp_eta=0.08, amp_phi=0.3):
-    """eta (smooth, two modes) and phi_s = Phi(x, y, eta) for a potential Phi
-    that solves Laplace's equation with a flat bed at -depth (deep if 0).
-    Returns the spectral state and the exact W = dPhi/dz at z = eta."""
-    x, y = grid(Nx, Ny, L, L)
-    k0 = 2 * np.pi / L
-    eta = amp_eta * (np.cos(k0 * x) + 0.5 * np.sin(k0 * (x + 2 * y)))
-    kx, ky = 2 * k0, k0
-    k = np.hypot(kx, ky)
-    th = kx * x + ky * y
-    if depth > 0:
-        Z = np.cosh(k * (eta + depth)) / np.cosh(k * depth)
-        dZ = k * np.sinh(k * (eta + depth)) / np.cosh(k * depth)
-    else:
-        Z = np.exp(k * eta)
-        dZ = k * np.exp(k * eta)
-    phi = amp_phi * Z * np.cos(th)
-    W = amp_phi * dZ * np.cos(th)
-    u = np.zeros((2, Ny // 2 + 1, Nx), complex)
-    u[0], u[1] = to_spec(eta), to_spec(phi)
-    return u, W, eta, phi
-
-
 @pytest.mark.parametrize("depth", [0.0, 0.8])
 def test_zvel_converges_to_exact_vertical_velocity(oracle_mod, depth):
     """W^(M) -> dPhi/dz(z=eta) with error falling by a factor ~k*eta per order."""
@@ -228,14 +205,7 @@ def test_rhs_converges_to_zakharov_with_exact_w(oracle_mod, depth):
     L = 2 * np.pi
     g = 9.81
     u, W, eta, phi = harmonic_state(Nx, Ny, L, depth, amp_eta=0.06, amp_phi=0.2)
-    k0 = np.fft.fftfreq(Nx, L / Nx / (2 * np.pi))
-    ex = np.real(np.fft.ifft(1j * k0[:, None] * np.fft.fft(eta, axis=0), axis=0))
-    ey = np.real(np.fft.ifft(1j * k0[None, :] * np.fft.fft(eta, axis=1), axis=1))
-    px = np.real(np.fft.ifft(1j * k0[:, None] * np.fft.fft(phi, axis=0), axis=0))
-    py = np.real(np.fft.ifft(1j * k0[None, :] * np.fft.fft(phi, axis=1), axis=1))
-    G = ex ** 2 + ey ** 2
-    et = -(ex * px + ey * py) + (1 + G) * W
-    pt = -g * eta - 0.5 * (px ** 2 + py ** 2) + 0.5 * (1 + G) * W * W
+    et, pt = zakharov_rhs(eta, phi, W, L, g)
     errs = []
     for nlevs in range(1, 6):
         o = oracle_mod.Oracle(Nx, Ny, nlevs, L, L, g=g, depth=depth)
@@ -245,6 +215,36 @@ def test_rhs_converges_to_zakharov_with_exact_w(oracle_mod, depth):
     for m in range(len(errs) - 1):
         assert errs[m + 1] < 0.5 * errs[m], errs
     assert errs[-1] < 1e-5, errs
+
+
+@pytest.mark.parametrize("depth", [0.0, 0.8])
+def test_truncation_order_in_amplitude(oracle_mod, depth):
+    """Scale eta and the harmonic potential by eps.  A sum truncated at order K
+    then misses its exact value by O(eps^(K+1)): W^(M), (W^2)^(M) and both RHS
+    components (against the Zakharov form with exact W) by eps^(M+1); W^(M-2)
+    by eps^(M-1) and (W^2)^(M-2) by eps^(M-1), where those sums are empty
+    (M <= 2) by W or W^2 itself, eps^1 and eps^2.  The error slopes between
+    eps = 1/4, 1/8 and 1/16 must match: a term dropped, mis-signed or
+    mis-weighted at order n <= K leaves a slope of n instead."""
+    Nx = Ny = 64
+    L = 2 * np.pi
+    g = 9.81
+    for nlevs in range(0, 5):
+        M = nlevs + 1
+        o = oracle_mod.Oracle(Nx, Ny, nlevs, L, L, g=g, depth=depth)
+        errs = []
+        for eps in (0.25, 0.125, 0.0625):
+            u, W, eta, phi = harmonic_state(Nx, Ny, L, depth, amp_eta=0.08 * eps, amp_phi=0.3 * eps)
+            wM, wM2, w2M, w2M2 = (o.ifft_2d(w) for w in o.Zvel(u))
+            et, pt = zakharov_rhs(eta, phi, W, L, g)
+            r = o.rhs_hos(u)
+            errs.append([np.abs(wM - W).max(), np.abs(wM2 - W).max(), np.abs(w2M - W * W).max(),
+                         np.abs(w2M2 - W * W).max(), np.abs(r[0] - o.Dealias(to_spec(et))).max(),
+                         np.abs(r[1] - o.Dealias(to_spec(pt))).max()])
+        errs = np.array(errs)
+        slopes = np.log2(errs[:-1] / errs[1:])
+        want = np.array([M + 1, max(M - 1, 1), M + 1, max(M - 1, 2), M + 1, M + 1])
+        assert np.all(np.abs(slopes - want) < 0.15), (M, slopes, want)
 
 
 # ---------------------------------------------------------------- RK (A1, A2)
