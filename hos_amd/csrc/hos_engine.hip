@@ -430,6 +430,7 @@ template <class T>
 static int launch_step(hos_ctx *c, int m0, int n, hipStream_t st)
 {
     Eng<T> e{c, n, m0, st};
+    e.inflight = c->active;  // the other member groups step concurrently
     const int nparts = x_nparts(c);
     const long s2 = 2 * c->fs;  // member stride of the FP64 [2][Nk][Nx] arrays
     int rc = HOS_OK;
