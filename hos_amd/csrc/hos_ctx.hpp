@@ -192,6 +192,9 @@ struct Eng {
     hipStream_t st = nullptr;  // launch stream (null: the context's)
     int inflight = 0;        // members whose launches run concurrently with this one (0: nm)
     hipStream_t S() const { return st ? st : c->stream; }
+    // enough spectral rows in flight to fill the GPU twice over and to outgrow
+    // its caches (8 members at 1024^2; one member at 1024^2 is not)
+    bool many_rows() const { return (long)(inflight > 0 ? inflight : nm) * c->Nk >= 4096; }
     cx<T> *I() const { return (cx<T> *)c->I + (long)m0 * sI(); }
     T *P() const { return (T *)c->P + (long)m0 * sP(); }
     // wavenumber tables of members m0.. (the kernels index them by blockIdx.z)
@@ -219,7 +222,7 @@ struct Eng {
     // source against 3.08 with chunks of 3
     int xinv(const cx<double> *Y, const std::vector<XinvOp> &list, int per = 0)
     {
-        if (per <= 0) per = ((long)(inflight > 0 ? inflight : nm) * c->Nk >= 4096) ? MAXPERGRP : 3;
+        if (per <= 0) per = many_rows() ? MAXPERGRP : 3;
         XinvOps ops{};
         for (int s = 0; s < 2; ++s) {
             int k = 0;

@@ -438,6 +438,7 @@ static int launch_step(hos_ctx *c, int m0, int n, hipStream_t st)
         RkArgs rk = rk_base(c);
         rk.stage = s;
         rk.c_s = c->c[s];
+        rk.stream = e.many_rows();
         rk.t += m0;
         rk.dt += m0;
         rk.U += m0 * s2;

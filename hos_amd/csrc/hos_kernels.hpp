@@ -82,6 +82,31 @@ __device__ __forceinline__ void store_nt(cx<T> *p, cx<T> v)
     }
 }
 
+// loads with the non-temporal hint, for data at its last use in a pass whose
+// working set is far larger than the caches: the RK epilogue's reads of the
+// RHS pair, the stage input and the accumulator when many rows are in flight
+// (RkArgs::stream; 8 x 1024^2 FP32 members: 108 -> 100 us per launch).  One
+// member's data partly stays in the last-level cache between passes, and
+// there the hint costs hits
+__device__ __forceinline__ cx<double> load_nt(const cx<double> *p)
+{
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const d2 v = __builtin_nontemporal_load(reinterpret_cast<const d2 *>(p));
+    return {v[0], v[1]};
+}
+template <class T>
+__device__ __forceinline__ void load2_nt(const cx<T> *p, cx<T> &a, cx<T> &b)
+{
+    if constexpr (sizeof(T) == 4) {
+        const vec_t<float> v = __builtin_nontemporal_load(reinterpret_cast<const vec_t<float> *>(p));
+        a = {v[0], v[1]};
+        b = {v[2], v[3]};
+    } else {
+        a = load_nt(p);
+        b = load_nt(p + 1);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // wavenumbers: kx from a table (reference or signed), ky = j*Ky0,
 // operators_par.c:50-71, 105-116, 168-192.
@@ -1140,6 +1165,7 @@ struct RkArgs {
     double *vol;      // [members]
     int mean_level;
     const double *mean0;  // [members]
+    int stream;           // 1: last-use loads with the non-temporal hint
 };
 
 // RHS at spectral point idx = (ky, kx) of member m from the forward-transformed
@@ -1157,7 +1183,8 @@ __device__ __forceinline__ void rk_epilogue(const RkArgs &rk, const KTab &kt, in
         k[fsd + idx] = {scale * double(nb.x), scale * double(nb.y)};
         return;
     }
-    const cx<double> ye = y[idx], yp = y[fsd + idx];
+    const cx<double> ye = rk.stream ? load_nt(y + idx) : y[idx];
+    const cx<double> yp = rk.stream ? load_nt(y + fsd + idx) : y[fsd + idx];
     const double kxv = kt.kx[kx], kyv = ky * kt.ky0, h = kt.h(m);
     // |k| tanh(|k| h), the expression of k_th_table (same value): computed,
     // not read, since this pass is bound by its FP64 traffic
@@ -1202,8 +1229,8 @@ __device__ __forceinline__ void rk_epilogue(const RkArgs &rk, const KTab &kt, in
         ae = ue;
         ap = up;
     } else {
-        ae = acc[idx];
-        ap = acc[fsd + idx];
+        ae = rk.stream ? load_nt(acc + idx) : acc[idx];
+        ap = rk.stream ? load_nt(acc + fsd + idx) : acc[fsd + idx];
     }
     ae = {ae.x + fc * ke.x, ae.y + fc * ke.y};
     ap = {ap.x + fc * kp.x, ap.y + fc * kp.y};
@@ -1242,7 +1269,11 @@ __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, lo
             const int e = threadIdx.x + q * NT;
             const long idx = (long)min(ky0 + e / NX, kt.nk - 1) * NX + e % NX;
             if (ilv) {
-                load2(ia + 2 * idx, va[q], vb[q]);  // the interleaved pair of k_yr2c_pair
+                // the interleaved pair of k_yr2c_pair
+                if (rk.stream)
+                    load2_nt(ia + 2 * idx, va[q], vb[q]);
+                else
+                    load2(ia + 2 * idx, va[q], vb[q]);
             } else {
                 va[q] = ia[idx];
                 vb[q] = ib[idx];
