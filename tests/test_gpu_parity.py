@@ -247,6 +247,38 @@ def test_members_match_alone(hos):
         assert np.array_equal(alone.get_state(), batch.get_state(i))
 
 
+def test_many_rows_batch_matches_alone(hos):
+    """With at least 4096 spectral rows in flight (8 members at 512 x 1024)
+    the batch takes its large-batch paths: the x inverse outputs in one group
+    per source, non-temporal stores in the wave x passes and non-temporal loads
+    in the RK epilogue, over two member groups.  Members stepped that way are
+    bit-identical to the same members stepped alone (one member: 513 rows,
+    outputs in groups of 3, plain loads), both with one row per wave."""
+    Nx, Ny, nlevs, nm = 512, 1024, 4, 8
+    print("many rows batch", flush=True)
+    p = hos.params(Nx=Nx, Ny=Ny, nlevs=nlevs, precision=1, depth=1.0, Lx=32 * np.pi, Ly=64 * np.pi)
+    states = [random_state(Nx, Ny, nlevs, amp=0.03, slope=3.0, seed=60 + i, Lx=32 * np.pi, Ly=64 * np.pi)
+              for i in range(nm)]
+    dts = 0.02 + 0.001 * np.arange(nm)
+    batch = hos.Solver(p, nm)
+    batch.set_tile(2, hos.HOS_TILE_XPASS)
+    batch.set_tile(2, hos.HOS_TILE_GROUPS)
+    batch.use_graph(True)
+    for i, u in enumerate(states):
+        batch.set_state(u, i)
+    batch.step(3, dts)
+    got = [(batch.get_state(i), batch.energy(i)) for i in (0, nm - 1)]
+    batch.close()
+    for (u_b, e_b), i in zip(got, (0, nm - 1)):
+        alone = hos.Solver(p, 1)
+        alone.set_tile(2, hos.HOS_TILE_XPASS)
+        alone.set_state(states[i])
+        alone.step(3, dts[i])
+        assert np.array_equal(alone.get_state(), u_b)
+        assert alone.energy(0) == e_b
+        alone.close()
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_member_groups_match_one_chain(hos, graph):
     """Member groups on their own streams (2, 3, 4) give bit-identical states
