@@ -764,7 +764,7 @@ struct C2rBatch {
     cx<T> A[PERL], B[PERL];
     __device__ __forceinline__ void load(const cx<T> *__restrict__ base, long fs, const int *slot, int x0, int nx)
     {
-        const int tid = opaque_tid();
+        const int tid = opaque_tid<NTT>();
 #pragma unroll
         for (int q = 0; q < PERL; ++q) {
             const int e = min(tid + q * NTT, NLB - 1);  // past the end: reload, unused
@@ -782,7 +782,7 @@ struct C2rBatch {
     __device__ __forceinline__ void fill(cx<T> *buf) const
     {
         using L = Lds<T, NY>;
-        const int tid = opaque_tid();
+        const int tid = opaque_tid<NTT>();
 #pragma unroll
         for (int q = 0; q < PERL; ++q) {
             const int e = tid + q * NTT;
