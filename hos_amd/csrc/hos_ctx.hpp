@@ -39,7 +39,7 @@ struct hos_ctx {
     hipEvent_t gfork = nullptr, gjoin[MAXG] = {};
     bool f32 = false;
     long fs = 0;    // complex elements per spectral field  (Nk*Nx)
-    long fsP = 0;   // real elements per physical field     (Nx*Ny)
+    long fsP = 0;   // real elements per physical field     (Nx*Ny + one spare row)
     hipStream_t stream = nullptr;
     // device memory
     cx<double> *U = nullptr, *A = nullptr, *Y = nullptr, *K = nullptr;
@@ -269,7 +269,7 @@ struct Eng {
         LOGN_SWITCH(c->logny, {
             TX_SWITCH(T, tx, {
                 dim3 grid(c->p.Nx / TXV, nf, nm);
-                Prof pf(c, S(), HOS_K_YC2R, (double)nm * nf * sizeof(T) * (2.0 * c->fs + c->fsP));
+                Prof pf(c, S(), HOS_K_YC2R, (double)nm * nf * sizeof(T) * (2.0 * c->fs + c->p.Nx * (double)c->p.Ny));
                 hipLaunchKernelGGL((k_yc2r<T, LG, TXV>), grid, dim3(NT), 0, S(), I(), sI(), c->fs, P(), sP(),
                                    c->fsP, fm, c->p.Nx, twy());
             })
@@ -283,7 +283,7 @@ struct Eng {
         LOGN_SWITCH(c->logny, {
             TX_SWITCH(T, tx, {
                 dim3 grid(c->p.Nx / TXV, 1, nm);
-                Prof pf(c, S(), HOS_K_YR2C, (double)nm * sizeof(T) * (nin * (double)c->fsP + 2.0 * c->fs));
+                Prof pf(c, S(), HOS_K_YR2C, (double)nm * sizeof(T) * (nin * (double)c->p.Nx * c->p.Ny + 2.0 * c->fs));
                 hipLaunchKernelGGL((k_yr2c<T, LG, TXV, PRO>), grid, dim3(NT), 0, S(), pro, I(), sI(), c->fs,
                                    dst, c->p.Nx, twy());
             })
@@ -298,7 +298,7 @@ struct Eng {
             TX_SWITCH(T, tx, {
                 constexpr int TXP = TXV / 2;
                 dim3 grid(c->p.Nx / TXP, 1, nm);
-                Prof pf(c, S(), HOS_K_YR2C_PAIR, (double)nm * sizeof(T) * (nin * (double)c->fsP + 4.0 * c->fs));
+                Prof pf(c, S(), HOS_K_YR2C_PAIR, (double)nm * sizeof(T) * (nin * (double)c->p.Nx * c->p.Ny + 4.0 * c->fs));
                 hipLaunchKernelGGL((k_yr2c_pair<T, LG, TXP, PRO>), grid, dim3(NT), 0, S(), pro, I(), sI(),
                                    c->fs, da, c->p.Nx, twy());
             })
@@ -455,7 +455,7 @@ struct Eng {
                 constexpr int TXP = (TXP0 > 2 && TXP0 * (1 << LG) / V > 2 * NT_RHS) ? 2 : TXP0;
                 dim3 grid(c->p.Nx / TXP, 1, nm);
                 Prof pf(c, S(), HOS_K_YR2C_PAIR,
-                        (double)nm * sizeof(T) * (nin * (double)c->fsP + 2.0 * (MO + 4) * c->fs + 4.0 * c->fs));
+                        (double)nm * sizeof(T) * (nin * (double)c->p.Nx * c->p.Ny + 2.0 * (MO + 4) * c->fs + 4.0 * c->fs));
                 hipLaunchKernelGGL((k_yr2c_rhs<T, LG, TXP, MO>), grid, dim3(NT_RHS), 0, S(), phys(), c->ix, (int)comp(),
                                    I(), sI(), c->fs, dfr, 0, c->p.Nx, twy());
             })
