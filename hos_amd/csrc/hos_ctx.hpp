@@ -39,6 +39,8 @@ struct hos_ctx {
     hipEvent_t gfork = nullptr, gjoin[MAXG] = {};
     bool f32 = false;
     long fs = 0;    // complex elements per spectral field  (Nk*Nx)
+    long ip = 0;    // row pitch of the x-intermediates I    (Nx + 16)
+    long fsI = 0;   // complex elements per x-intermediate   (Nk*ip)
     long fsP = 0;   // real elements per physical field     (Nx*Ny + one spare row)
     hipStream_t stream = nullptr;
     // device memory
@@ -205,7 +207,7 @@ struct Eng {
         k.th += (long)m0 * k.ths;
         return k;
     }
-    long sI() const { return (long)c->nI * c->fs; }
+    long sI() const { return (long)c->nI * c->fsI; }
     long sP() const { return (long)c->nP * c->fsP; }
     long sY() const { return 2 * c->fs; }
     const cx<T> *twx() const { return (const cx<T> *)c->twx; }
@@ -251,13 +253,13 @@ struct Eng {
             if constexpr (x_wave<T>(LG)) {
                 if (c->xwave) {
                     dim3 gw((c->Nk + XW - 1) / XW, ngroups, nm);
-                    hipLaunchKernelGGL((k_xinv_w<T, LG>), gw, dim3(NT), 0, S(), Y, sY(), I(), sI(), c->fs, ops,
+                    hipLaunchKernelGGL((k_xinv_w<T, LG>), gw, dim3(NT), 0, S(), Y, sY(), I(), sI(), c->fs, c->fsI, c->ip, ops,
                                        kt(), twx());
                     waved = true;
                 }
             }
             if (!waved) {
-                hipLaunchKernelGGL((k_xinv<T, LG, ROWS>), grid, dim3(NT), 0, S(), Y, sY(), I(), sI(), c->fs,
+                hipLaunchKernelGGL((k_xinv<T, LG, ROWS>), grid, dim3(NT), 0, S(), Y, sY(), I(), sI(), c->fs, c->fsI, c->ip,
                                    ops, kt(), twx());
             }
         });
@@ -270,8 +272,8 @@ struct Eng {
             TX_SWITCH(T, tx, {
                 dim3 grid(c->p.Nx / TXV, nf, nm);
                 Prof pf(c, S(), HOS_K_YC2R, (double)nm * nf * sizeof(T) * (2.0 * c->fs + c->p.Nx * (double)c->p.Ny));
-                hipLaunchKernelGGL((k_yc2r<T, LG, TXV>), grid, dim3(NT), 0, S(), I(), sI(), c->fs, P(), sP(),
-                                   c->fsP, fm, c->p.Nx, twy());
+                hipLaunchKernelGGL((k_yc2r<T, LG, TXV>), grid, dim3(NT), 0, S(), I(), sI(), c->fsI, P(), sP(),
+                                   c->fsP, fm, (int)c->ip, twy());
             })
         });
         return HOS_OK;
@@ -284,8 +286,8 @@ struct Eng {
             TX_SWITCH(T, tx, {
                 dim3 grid(c->p.Nx / TXV, 1, nm);
                 Prof pf(c, S(), HOS_K_YR2C, (double)nm * sizeof(T) * (nin * (double)c->p.Nx * c->p.Ny + 2.0 * c->fs));
-                hipLaunchKernelGGL((k_yr2c<T, LG, TXV, PRO>), grid, dim3(NT), 0, S(), pro, I(), sI(), c->fs,
-                                   dst, c->p.Nx, twy());
+                hipLaunchKernelGGL((k_yr2c<T, LG, TXV, PRO>), grid, dim3(NT), 0, S(), pro, I(), sI(), c->fsI,
+                                   dst, (int)c->ip, twy());
             })
         });
         return HOS_OK;
@@ -300,7 +302,7 @@ struct Eng {
                 dim3 grid(c->p.Nx / TXP, 1, nm);
                 Prof pf(c, S(), HOS_K_YR2C_PAIR, (double)nm * sizeof(T) * (nin * (double)c->p.Nx * c->p.Ny + 4.0 * c->fs));
                 hipLaunchKernelGGL((k_yr2c_pair<T, LG, TXP, PRO>), grid, dim3(NT), 0, S(), pro, I(), sI(),
-                                   c->fs, da, c->p.Nx, twy());
+                                   c->fsI, da, (int)c->ip, twy());
             })
         });
         return HOS_OK;
@@ -317,13 +319,13 @@ struct Eng {
             if constexpr (x_wave<T>(LG)) {
                 if (c->xwave) {
                     dim3 gw((c->Nk + XW - 1) / XW, 1, nm);
-                    hipLaunchKernelGGL((k_xfwdinv_w<T, LG>), gw, dim3(NT), 0, S(), I(), sI(), c->fs, src, dst0,
+                    hipLaunchKernelGGL((k_xfwdinv_w<T, LG>), gw, dim3(NT), 0, S(), I(), sI(), c->fsI, c->ip, src, dst0,
                                        ndst, dtop, scale, kt(), twx());
                     waved = true;
                 }
             }
             if (!waved) {
-                hipLaunchKernelGGL((k_xfwdinv<T, LG, ROWS>), grid, dim3(NT), 0, S(), I(), sI(), c->fs, src, dst0,
+                hipLaunchKernelGGL((k_xfwdinv<T, LG, ROWS>), grid, dim3(NT), 0, S(), I(), sI(), c->fsI, c->ip, src, dst0,
                                    ndst, dtop, scale, kt(), twx());
             }
         });
@@ -348,7 +350,7 @@ struct Eng {
             constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
             dim3 grid((c->Nk + ROWS - 1) / ROWS, 1, nm);
             Prof pf(c, S(), HOS_K_XFWD_RHS, (double)nm * c->fs * (2.0 * 2.0 * sizeof(T) + 16.0 * f64));
-            hipLaunchKernelGGL((k_xfwd_rhs<T, LG, ROWS>), grid, dim3(NT), 0, S(), I(), sI(), c->fs, ia,
+            hipLaunchKernelGGL((k_xfwd_rhs<T, LG, ROWS>), grid, dim3(NT), 0, S(), I(), sI(), c->fsI, c->ip, ia,
                                ib < 0 ? ia : ib, ib < 0 ? 1 : 0, Y, sY(), scale, kt(), rk, twx());
         });
         return HOS_OK;
@@ -457,7 +459,7 @@ struct Eng {
                 Prof pf(c, S(), HOS_K_YR2C_PAIR,
                         (double)nm * sizeof(T) * (nin * (double)c->p.Nx * c->p.Ny + 2.0 * (MO + 4) * c->fs + 4.0 * c->fs));
                 hipLaunchKernelGGL((k_yr2c_rhs<T, LG, TXP, MO>), grid, dim3(NT_RHS), 0, S(), phys(), c->ix, (int)comp(),
-                                   I(), sI(), c->fs, dfr, 0, c->p.Nx, twy());
+                                   I(), sI(), c->fsI, dfr, 0, (int)c->ip, twy());
             })
         });
         return HOS_OK;

@@ -205,7 +205,7 @@ struct XGeom {
 // group is formed from it.  grid (ceil(Nk/ROWS), ngroups, members)
 template <class T, int LOGNX, int ROWS>
 __global__ __launch_bounds__(NT) void k_xinv(const cx<double> *__restrict__ Y, long sY, cx<T> *__restrict__ I,
-                                             long sI, long fs, XinvOps ops, KTab kt,
+                                             long sI, long fs, long fsI, long ip, XinvOps ops, KTab kt,
                                              const cx<T> *__restrict__ twx)
 {
     using G = XGeom<T, LOGNX>;
@@ -274,12 +274,12 @@ __global__ __launch_bounds__(NT) void k_xinv(const cx<double> *__restrict__ Y, l
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             if (o0 + b >= nops) break;
-            cx<T> *dst = base + ops.o[grp][o0 + b].dst * fs;
+            cx<T> *dst = base + ops.o[grp][o0 + b].dst * fsI;
 #pragma unroll
             for (int q = 0; q < PER; ++q) {
                 const int e = threadIdx.x + q * NT;
                 const int kx = e % NX, ky = ky0 + e / NX;
-                if (ky < kt.nk) dst[(long)ky * NX + kx] = buf[L::at(b * ROWS + e / NX, kx)];
+                if (ky < kt.nk) dst[(long)ky * ip + kx] = buf[L::at(b * ROWS + e / NX, kx)];
             }
         }
     }
@@ -1071,8 +1071,9 @@ __global__ __launch_bounds__(NT_RHS) void k_yr2c_rhs(Phys<T> ph, HosIdx ix, int 
 // thread keeps its modes in registers; the inverses run XNB rows at a time).
 // grid (ceil(Nk/ROWS), 1, members)
 template <class T, int LOGNX, int ROWS>
-__global__ __launch_bounds__(NT) void k_xfwdinv(cx<T> *__restrict__ I, long sI, long fs, int src, int dst0, int ndst,
-                                                int dtop, double scale, KTab kt, const cx<T> *__restrict__ twx)
+__global__ __launch_bounds__(NT) void k_xfwdinv(cx<T> *__restrict__ I, long sI, long fs, long ip, int src, int dst0,
+                                                int ndst, int dtop, double scale, KTab kt,
+                                                const cx<T> *__restrict__ twx)
 {
     using G = XGeom<T, LOGNX>;
     static_assert(G::ROWS == ROWS, "x rows");
@@ -1091,7 +1092,7 @@ __global__ __launch_bounds__(NT) void k_xfwdinv(cx<T> *__restrict__ I, long sI, 
     for (int q = 0; q < PER; ++q) {
         const int e = threadIdx.x + q * NT;
         const int kx = e % NX, ky = min(ky0 + e / NX, kt.nk - 1);
-        s[q] = in[(long)ky * NX + kx];
+        s[q] = in[(long)ky * ip + kx];
         kxv[q] = kt.kx[kx];
         if constexpr (sizeof(R) == 8) thd[q] = tht[(long)ky * NX + kx];
         else thd[q] = 1.0;
@@ -1135,7 +1136,7 @@ __global__ __launch_bounds__(NT) void k_xfwdinv(cx<T> *__restrict__ I, long sI, 
             for (int q = 0; q < PER; ++q) {
                 const int e = threadIdx.x + q * NT;
                 const int kx = e % NX, ky = ky0 + e / NX;
-                if (ky < kt.nk) out[(long)ky * NX + kx] = W[L::at(b * ROWS + e / NX, kx)];
+                if (ky < kt.nk) out[(long)ky * ip + kx] = W[L::at(b * ROWS + e / NX, kx)];
             }
         }
     }
@@ -1248,9 +1249,9 @@ __device__ __forceinline__ void rk_epilogue(const RkArgs &rk, const KTab &kt, in
 }
 
 template <class T, int LOGNX, int ROWS>
-__global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, long sI, long fs, int iA, int iB,
-                                                 int ilv, const cx<double> *Y, long sY, double scale, KTab kt,
-                                                 RkArgs rk, const cx<T> *__restrict__ twx)
+__global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, long sI, long fs, long ip, int iA,
+                                                 int iB, int ilv, const cx<double> *Y, long sY, double scale,
+                                                 KTab kt, RkArgs rk, const cx<T> *__restrict__ twx)
 {
     using G = XGeom<T, LOGNX>;
     static_assert(G::ROWS == ROWS, "x rows");
@@ -1267,7 +1268,7 @@ __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, lo
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int e = threadIdx.x + q * NT;
-            const long idx = (long)min(ky0 + e / NX, kt.nk - 1) * NX + e % NX;
+            const long idx = (long)min(ky0 + e / NX, kt.nk - 1) * ip + e % NX;  // x-intermediate row
             if (ilv) {
                 // the interleaved pair of k_yr2c_pair
                 if (rk.stream)
@@ -1330,7 +1331,8 @@ constexpr int XW = NT / 64;
 
 template <class T, int LOGNX>
 __global__ __launch_bounds__(NT) void k_xinv_w(const cx<double> *__restrict__ Y, long sY, cx<T> *__restrict__ I,
-                                               long sI, long fs, XinvOps ops, KTab kt, const cx<T> *__restrict__ twx)
+                                               long sI, long fs, long fsI, long ip, XinvOps ops, KTab kt,
+                                               const cx<T> *__restrict__ twx)
 {
     constexpr int NX = 1 << LOGNX, E = NX / 64;
     using L = Lds<T, NX>;
@@ -1391,7 +1393,7 @@ __global__ __launch_bounds__(NT) void k_xinv_w(const cx<double> *__restrict__ Y,
             for (int i = 0; i < E; ++i) v[i] = {T(sr[i].x), T(sr[i].y)};
         }
         wave_fft<T, LOGNX, true>(v, row, twx, lane);
-        cx<T> *dst = I + m * sI + op.dst * fs + r0;
+        cx<T> *dst = I + m * sI + op.dst * fsI + (long)ky * ip;
 #pragma unroll
         for (int i = 0; i < E; ++i) store_nt(dst + lane + 64 * i, v[i]);
     }
@@ -1400,8 +1402,8 @@ __global__ __launch_bounds__(NT) void k_xinv_w(const cx<double> *__restrict__ Y,
 // x forward pass of phi^(lev+1), d^j/dz^j for j = 1..ndst and the x inverse of
 // each, one row per wave: the spectral row stays in registers throughout
 template <class T, int LOGNX>
-__global__ __launch_bounds__(NT) void k_xfwdinv_w(cx<T> *__restrict__ I, long sI, long fs, int src, int dst0,
-                                                  int ndst, int dtop, double scale, KTab kt,
+__global__ __launch_bounds__(NT) void k_xfwdinv_w(cx<T> *__restrict__ I, long sI, long fs, long ip, int src,
+                                                  int dst0, int ndst, int dtop, double scale, KTab kt,
                                                   const cx<T> *__restrict__ twx)
 {
     constexpr int NX = 1 << LOGNX, E = NX / 64;
@@ -1413,9 +1415,9 @@ __global__ __launch_bounds__(NT) void k_xfwdinv_w(cx<T> *__restrict__ I, long sI
     if (ky >= kt.nk) return;
     cx<T> *row = lds + w * L::PITCH;
     const int m = blockIdx.z;
-    const long r0 = (long)ky * NX;
+    const long r0 = (long)ky * NX, rI = (long)ky * ip;  // tanh table row, x-intermediate row
     cx<T> *base = I + m * sI;
-    const cx<T> *in = base + src * fs + r0;
+    const cx<T> *in = base + src * fs + rI;
     const double *tht = kt.th + m * kt.ths + r0;
     const double h = kt.h(m), kyv = ky * kt.ky0;
     cx<T> s[E];
@@ -1446,7 +1448,7 @@ __global__ __launch_bounds__(NT) void k_xfwdinv_w(cx<T> *__restrict__ I, long sI
             v[i] = {mul * s[i].x, mul * s[i].y};
         }
         wave_fft<T, LOGNX, true, true>(v, row, twx, lane);
-        cx<T> *out = base + ((j == ndst && dtop >= 0) ? dtop : dst0 + j - 1) * fs + r0;
+        cx<T> *out = base + ((j == ndst && dtop >= 0) ? dtop : dst0 + j - 1) * fs + rI;
 #pragma unroll
         for (int i = 0; i < E; ++i) store_nt(out + lane + 64 * i, v[i]);
     }
