@@ -44,7 +44,7 @@ EXPORTS = [
     "hos_last_error", "hos_device_name", "hos_num_members", "hos_bytes_per_point_step", "hos_step_times",
     "hos_profile", "hos_kernel_stats", "hos_spectral_op", "hos_mult", "hos_hamiltonian",
     "hos_set_active", "hos_get_active", "hos_set_member", "hos_copy_member", "hos_step_async", "hos_wait",
-    "hos_autotune", "hos_copy_bandwidth", "hos_get_tile",
+    "hos_autotune", "hos_copy_bandwidth", "hos_get_tile", "hos_query",
 ]
 HOS_TILE_C2R, HOS_TILE_R2C, HOS_TILE_PAIR, HOS_TILE_ALL, HOS_TILE_XPASS, HOS_TILE_GROUPS = 0, 1, 2, 3, 4, 5
 HOS_OP_DX, HOS_OP_DY, HOS_OP_DZ, HOS_OP_SUM, HOS_OP_DEALIAS = 1, 2, 3, 4, 5
@@ -118,6 +118,7 @@ def load():
     lib.hos_copy_member.argtypes = [vp, ip, ip]
     lib.hos_step_async.argtypes = [vp, ip]
     lib.hos_wait.argtypes = [vp]
+    lib.hos_query.argtypes = [vp]
     lib.hos_ic_jonswap.argtypes = [ctypes.POINTER(Jonswap), vp]
     lib.hos_ic_uniform.argtypes = [ctypes.c_ulonglong, ip, ip]
     lib.hos_ic_uniform.restype = ctypes.c_double
@@ -188,7 +189,7 @@ def jonswap(Nx, Ny, Lx, Ly, kp=1.0, gamma=3.3, eps=0.1, depth=0.0, g=9.81, theta
 
 class ServerParams(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_double), ("precision", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("max_slots", ctypes.c_int), ("compensated", ctypes.c_int)]
+                ("max_slots", ctypes.c_int), ("compensated", ctypes.c_int), ("steps_per_launch", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):
@@ -199,14 +200,15 @@ class Request(ctypes.Structure):
 
 SERVE_EXPORTS = ["hos_server_create", "hos_server_destroy", "hos_server_submit", "hos_server_status",
                  "hos_server_result", "hos_server_step", "hos_server_start", "hos_server_stop",
-                 "hos_server_drain", "hos_server_stats"]
+                 "hos_server_drain", "hos_server_stats", "hos_server_fail_launches"]
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
 
 class Server:
     """hos_server (include/hos_serve.h): continuous batching of many runs."""
 
-    def __init__(self, hbm_budget=250e9, precision=HOS_FP32, use_graph=True, max_slots=0, compensated=True):
+    def __init__(self, hbm_budget=250e9, precision=HOS_FP32, use_graph=True, max_slots=0, compensated=True,
+                 steps_per_launch=0):
         lib = load()
         vp = ctypes.c_void_p
         lib.hos_server_create.argtypes = [ctypes.POINTER(ServerParams), ctypes.POINTER(vp)]
@@ -220,7 +222,8 @@ class Server:
         lib.hos_server_drain.argtypes = [vp, ctypes.c_double]
         lib.hos_server_stats.argtypes = [vp, vp, ctypes.c_int]
         self.lib = lib
-        sp = ServerParams(float(hbm_budget), precision, int(use_graph), max_slots, int(compensated))
+        sp = ServerParams(float(hbm_budget), precision, int(use_graph), max_slots, int(compensated),
+                          int(steps_per_launch))
         h = vp()
         if lib.hos_server_create(ctypes.byref(sp), ctypes.byref(h)) != 0:
             raise HosError("hos_server_create rejected its parameters")
@@ -279,6 +282,11 @@ class Server:
             return False
         _check(rc)
         return True
+
+    def fail_launches(self, n: int):
+        """Failure injection for tests: the next n pool launches fail."""
+        self.lib.hos_server_fail_launches.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        _check(self.lib.hos_server_fail_launches(self.h, n))
 
     def stats(self) -> dict:
         v = np.zeros(12)
@@ -390,6 +398,13 @@ class Solver:
 
     def wait(self):
         _check(self.lib.hos_wait(self.h))
+
+    def query(self) -> bool:
+        """hos_query: True while steps of step_async are still running."""
+        r = self.lib.hos_query(self.h)
+        if r < 0:
+            raise HosError(self.lib.hos_last_error().decode())
+        return r == 1
 
     def spectral_op(self, op: int, a, b=None) -> np.ndarray:
         a = np.ascontiguousarray(a, dtype=np.complex128)

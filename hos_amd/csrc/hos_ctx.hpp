@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <initializer_list>
 #include <vector>
 
 #include "../../include/hos_gpu.h"
@@ -186,6 +187,26 @@ inline int x_nparts(const hos_ctx *c)
     return (c->Nk + rows - 1) / rows;
 }
 
+// the outputs of one x inverse pass, in order (fixed capacity: the launch
+// path allocates nothing)
+struct XinvList {
+    XinvOp v[MAXOPS];
+    int n = 0;
+    XinvList() = default;
+    XinvList(std::initializer_list<XinvOp> l)
+    {
+        for (const XinvOp &o : l) push_back(o);
+    }
+    void push_back(const XinvOp &o)
+    {
+        if (n < MAXOPS) v[n] = o;
+        ++n;  // past MAXOPS: rejected by Eng::xinv
+    }
+    int size() const { return n; }
+    const XinvOp *begin() const { return v; }
+    const XinvOp *end() const { return v + (n < MAXOPS ? n : MAXOPS); }
+};
+
 template <class T>
 struct Eng {
     hos_ctx *c;
@@ -222,8 +243,9 @@ struct Eng {
     // flight count every member group stepping at the same time: two groups
     // of four 1024^2 members measured 3.01 ms per step with one chunk per
     // source against 3.08 with chunks of 3
-    int xinv(const cx<double> *Y, const std::vector<XinvOp> &list, int per = 0)
+    int xinv(const cx<double> *Y, const XinvList &list, int per = 0)
     {
+        if (list.size() > MAXOPS) return fail(HOS_E_INVALID, "too many x-inverse outputs");
         if (per <= 0) per = many_rows() ? MAXPERGRP : 3;
         XinvOps ops{};
         for (int s = 0; s < 2; ++s) {
@@ -366,7 +388,7 @@ struct Eng {
     // outputs of the first x inverse pass; the ones bound for the physical
     // grid are listed in fm, their count returned.  defer: eta_x, eta_y,
     // phi_x, phi_y and d^M phi/dz^M stay x-intermediates for the RHS pass.
-    int level0(std::vector<XinvOp> &ops, FieldMap &fm, bool defer)
+    int level0(XinvList &ops, FieldMap &fm, bool defer)
     {
         const HosIdx &ix = c->ix;
         const int M = c->M;
@@ -406,7 +428,7 @@ struct Eng {
     {
         const HosIdx &ix = c->ix;
         const int M = c->M;
-        std::vector<XinvOp> ops;
+        XinvList ops;
         FieldMap fm{};
         const int n = level0(ops, fm, defer);
         int rc = xinv(Y, ops);

@@ -477,7 +477,8 @@ static int launch_step(hos_ctx *c, int m0, int n, hipStream_t st)
             dim3 grid((unsigned)std::min<long>((s2 + NT - 1) / NT, 2048), n);
             if (!rc) {
                 hipLaunchKernelGGL(k_combine, grid, dim3(NT), 0, e.S(), c->U, c->K, (long)c->nm * s2, s2,
-                                   last ? c->U : c->Y, s2, c->dt, ca);
+                                   last ? c->U : c->Y, s2, c->dt, ca,
+                                   (last && c->p.mean_level) ? (const double *)c->mean0 : nullptr);
                 if (last)
                     hipLaunchKernelGGL(k_copy_c64, dim3(2048), dim3(NT), 0, e.S(), c->U, c->Y, (long)n * s2);
             }
@@ -675,6 +676,16 @@ extern "C" int hos_wait(hos_ctx *c)
     return step_finish(c);
 }
 
+extern "C" int hos_query(hos_ctx *c)
+{
+    if (!c) return fail(HOS_E_INVALID, "null context");
+    if (!c->async_pending) return 0;
+    const hipError_t e = hipEventQuery(c->ev1);
+    if (e == hipErrorNotReady) return 1;
+    if (e != hipSuccess) return fail(HOS_E_DEVICE, "hos_query: %s", hipGetErrorString(e));
+    return step_finish(c);
+}
+
 extern "C" int hos_set_active(hos_ctx *c, int n)
 {
     if (!c) return fail(HOS_E_INVALID, "null context");
@@ -840,15 +851,17 @@ extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
             return HOS_OK;
         }
     }
-    // save what a trial step changes: U, Y, t, H, volume
+    // save what a trial step changes: U, Y, t, dt, H, volume
     const size_t st = (size_t)c->nm * 2 * c->fs * sizeof(cx<double>);
+    std::vector<double> t_save = c->host_t, dt_save = c->host_dt, h_save(c->nm), vol_save(c->nm);
+    HIP_TRY(hipMemcpy(h_save.data(), c->H, sizeof(double) * c->nm, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(vol_save.data(), c->vol, sizeof(double) * c->nm, hipMemcpyDeviceToHost));
     void *saveU = nullptr, *saveY = nullptr;
     HIP_TRY(hipMalloc(&saveU, st));
     if (hipMalloc(&saveY, st) != hipSuccess) {
         (void)hipFree(saveU);
         return fail(HOS_E_NOMEM, "autotune scratch");
     }
-    std::vector<double> t_save = c->host_t;
     (void)hipMemcpyAsync(saveU, c->U, st, hipMemcpyDeviceToDevice, c->stream);
     (void)hipMemcpyAsync(saveY, c->Y, st, hipMemcpyDeviceToDevice, c->stream);
     std::vector<double> dtev(c->nm);
@@ -929,6 +942,12 @@ extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
     (void)hipFree(saveY);
     c->host_t = t_save;
     (void)hipMemcpy(c->t, c->host_t.data(), sizeof(double) * c->nm, hipMemcpyHostToDevice);
+    (void)hipMemcpy(c->H, h_save.data(), sizeof(double) * c->nm, hipMemcpyHostToDevice);
+    (void)hipMemcpy(c->vol, vol_save.data(), sizeof(double) * c->nm, hipMemcpyHostToDevice);
+    {
+        const int rd = upload_dt(c, dt_save.data());
+        if (!rc) rc = rd;
+    }
     if (rc) return rc;
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
@@ -1112,7 +1131,7 @@ template <class T>
 static int ifft_impl(hos_ctx *c)
 {
     Eng<T> e{c, 1};
-    const std::vector<XinvOp> ops = {{0, OP_ID, 0, 0}};
+    const XinvList ops = {{0, OP_ID, 0, 0}};
     FieldMap fm{};
     fm.src[0] = 0;
     fm.dst[0] = 0;
@@ -1144,7 +1163,7 @@ template <class T>
 static int mult_impl(hos_ctx *c)
 {
     Eng<T> e{c, 1};
-    const std::vector<XinvOp> ops = {{0, OP_ID, 0, 0}, {1, OP_ID, 0, 1}};
+    const XinvList ops = {{0, OP_ID, 0, 0}, {1, OP_ID, 0, 1}};
     FieldMap fm{};
     fm.src[0] = 0;
     fm.dst[0] = 0;

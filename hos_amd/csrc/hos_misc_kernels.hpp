@@ -43,14 +43,17 @@ __global__ __launch_bounds__(NT) void k_end_step(const double *__restrict__ hpar
 }
 
 // generic Butcher combination for schemes with a full lower triangle:
-// out = u + dt * sum_p coef[p] K_p   (time_schemes_par.c:38-67, 98-116)
+// out = u + dt * sum_p coef[p] K_p   (time_schemes_par.c:38-67, 98-116).
+// mean0 (non-null on the last stage with the mean-level correction on): the
+// new state's eta-hat(0,0) is held at mean0[m], as the sub-diagonal epilogue
+// does.
 struct CombArgs {
     double coef[8];
     int np;
 };
 __global__ __launch_bounds__(NT) void k_combine(const cx<double> *U, const cx<double> *__restrict__ K,
                                                 long sK_stage, long sM, cx<double> *out, long n, const double *dt,
-                                                CombArgs ca)
+                                                CombArgs ca, const double *mean0)
 {
     const int m = blockIdx.y;
     const double d = dt[m];
@@ -61,6 +64,7 @@ __global__ __launch_bounds__(NT) void k_combine(const cx<double> *U, const cx<do
             v.x += d * ca.coef[p] * k.x;
             v.y += d * ca.coef[p] * k.y;
         }
+        if (mean0 && i == 0) v = {mean0[m], 0.0};
         out[m * sM + i] = v;
     }
 }

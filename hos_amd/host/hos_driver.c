@@ -23,7 +23,9 @@
  * The reference hard-codes scheme 4 and dt = 0.2 (main_2d_par.c:252,302) and
  * NLevs at compile time; here they are options with those defaults:
  *   hos_driver [-d dir] [-n nlevs] [-s scheme] [-t dt] [-h depth] [-f32]
- *              [-q kx_quirk] [-N max_steps]
+ *              [-q kx_quirk] [-N max_steps] [-S saveflg]
+ * (-S overrides the saveflg read from initpars, e.g. to write the data_extra
+ * files from an input pair that sets saveflg = 1.)
  */
 #include <hdf5.h>
 #include <math.h>
@@ -257,7 +259,7 @@ static hid_t snapshot(const char *fn, double t)
 
 int main(int argc, char **argv)
 {
-    int scheme = 4, nmax = -1, info = 0;
+    int scheme = 4, nmax = -1, info = 0, save_override = -1;
     double dt = 0.2;
     const char *dir = NULL;
     for (int a = 1; a < argc; a++) {
@@ -269,9 +271,10 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[a], "-h") && a + 1 < argc) hos_depth = atof(argv[++a]);
         else if (!strcmp(argv[a], "-q") && a + 1 < argc) hos_kx_quirk = atoi(argv[++a]);
         else if (!strcmp(argv[a], "-N") && a + 1 < argc) nmax = atoi(argv[++a]);
+        else if (!strcmp(argv[a], "-S") && a + 1 < argc) save_override = atoi(argv[++a]);
         else if (!strcmp(argv[a], "-f32")) hos_precision = HOS_FP32;
         else {
-            fprintf(stderr, "usage: %s [-d dir] [-n nlevs] [-s scheme] [-t dt] [-h depth] [-f32] [-q 0|1] [-N steps]\n",
+            fprintf(stderr, "usage: %s [-d dir] [-n nlevs] [-s scheme] [-t dt] [-h depth] [-f32] [-q 0|1] [-N steps] [-S saveflg]\n",
                     argv[0]);
             return 2;
         }
@@ -289,6 +292,7 @@ int main(int argc, char **argv)
         fprintf(stderr, "neither initpars.h5 nor initpars.txt found\n");
         return 1;
     }
+    if (save_override >= 0) saveflg = save_override;
     char subid[2], name[128];
     snprintf(subid, 2, "%d", runsubid); /* one digit, main_2d_par.c:93 */
     snprintf(name, sizeof name, "initdata.%s.h5", subid);

@@ -167,6 +167,8 @@ void sol_update_RK(hos_cplx *u, double *t, double dt, char *dtflag)
 /* ---------------------------------------------------------------- L3 */
 void rhs_hos_setup(void)
 {
+    /* a state still on the GPU is copied back before its context goes */
+    hos_ref_sync();
     teardown();
     ensure_ctx();
 }
@@ -189,10 +191,13 @@ void rhs_hos(hos_cplx *hrhs, hos_cplx *hu, double t)
 void Zvel(hos_cplx *hu, hos_cplx *hZvelM, hos_cplx *hZvelM2, hos_cplx *hZvel2M, hos_cplx *hZvel2M2, double t)
 {
     (void)t;
+    hos_cplx *outs[4] = {hZvelM, hZvelM2, hZvel2M, hZvel2M2};
     ensure_ctx();
     touch(hu);
+    for (int q = 0; q < 4; q++) touch(outs[q]);
     CHECK(hos_zvel(ctx, (const double *)hu, (double *)hZvelM, (double *)hZvelM2, (double *)hZvel2M,
                    (double *)hZvel2M2));
+    for (int q = 0; q < 4; q++) wrote(outs[q]);
 }
 
 void ZvelLinear(const hos_cplx *hu, hos_cplx *out)

@@ -32,7 +32,10 @@ struct Req {
 struct Pool {
     hos_request key{};
     hos_ctx *ctx = nullptr;
-    int cap = 0, active = 0, idle = 0;
+    int cap = 0, active = 0;
+    int inflight = 0;        // steps enqueued by the last launch and not yet collected
+    bool broken = false;     // a launch or its steps failed: no more admissions
+    double empty_since = 0;  // server time the pool last became empty
     size_t bytes = 0;
     std::vector<long long> slot_req;
     std::vector<int> remaining;
@@ -71,6 +74,9 @@ struct hos_server {
     std::vector<double> step_ms, lat, qwait;
     double pt_steps = 0, busy = 0;
     int pools_created = 0, peak_active = 0, running = 0;
+    double t_last = 0;        // time of the previous scheduler iteration
+    int inject_fail = 0;      // pool launches still to fail (hos_server_fail_launches)
+    bool any_inflight = false; // a pool had steps in flight after it
 
     double now() const { return std::chrono::duration<double>(clk::now() - t0).count(); }
 };
@@ -102,10 +108,12 @@ static int default_cap(const hos_server *s, const hos_request &r)
 extern "C" int hos_server_create(const hos_server_params *p, hos_server **out)
 {
     if (!p || !out) return HOS_E_INVALID;
-    if (!(p->hbm_budget > 0) || (p->precision != HOS_FP32 && p->precision != HOS_FP64) || p->max_slots < 0)
+    if (!(p->hbm_budget > 0) || (p->precision != HOS_FP32 && p->precision != HOS_FP64) || p->max_slots < 0 ||
+        p->steps_per_launch < 0)
         return HOS_E_INVALID;
     hos_server *s = new hos_server();
     s->p = *p;
+    if (s->p.steps_per_launch == 0) s->p.steps_per_launch = 8;
     *out = s;
     return HOS_OK;
 }
@@ -162,58 +170,162 @@ extern "C" int hos_server_result(hos_server *s, long long id, double *u, double 
     return st == HOS_REQ_DONE ? HOS_OK : HOS_E_DEVICE;
 }
 
-// find a pool with a free slot for r, creating one if the budget allows
+static void destroy_pool(hos_server *s, size_t i)
+{
+    Pool *p = s->pools[i];
+    hos_destroy(p->ctx);
+    s->used -= p->bytes;
+    delete p;
+    s->pools.erase(s->pools.begin() + (long)i);
+}
+
+// release empty pools: all of them, or those empty for longer than a second
+static void release_idle(hos_server *s, bool all)
+{
+    const double now = s->now();
+    for (size_t i = 0; i < s->pools.size();) {
+        Pool *p = s->pools[i];
+        if (p->active == 0 && p->inflight == 0 && (all || p->broken || now - p->empty_since > 1.0))
+            destroy_pool(s, i);
+        else
+            ++i;
+    }
+}
+
+// find an idle pool with a free slot for r, creating one if the budget allows
+// (after releasing empty pools, if it does not at first); nullptr: wait
 static Pool *pool_for(hos_server *s, const hos_request &r)
 {
-    for (Pool *p : s->pools)
-        if (same_shape(p->key, r) && p->active < p->cap) return p;
-    for (Pool *p : s->pools)
-        if (same_shape(p->key, r)) return nullptr;  // full: wait for a slot
+    bool same_busy = false, same_full = false;
+    for (Pool *p : s->pools) {
+        if (!same_shape(p->key, r) || p->broken) continue;
+        if (p->inflight > 0) {
+            same_busy |= p->active < p->cap;
+            continue;
+        }
+        if (p->active < p->cap) return p;
+        same_full = true;
+    }
+    if (same_busy || same_full) return nullptr;  // a slot of this shape frees up soon
     hos_params pp = pool_params(s, r);
-    for (int cap = default_cap(s, r); cap >= 1; cap /= 2) {
-        const size_t b = hos_footprint(&pp, cap);
-        if (b == 0 || s->used + b > (size_t)s->p.hbm_budget) continue;
-        hos_ctx *c = nullptr;
-        if (hos_create(&pp, cap, &c) != HOS_OK) return nullptr;
-        hos_use_graph(c, s->p.use_graph);
-        hos_set_active(c, 0);
-        Pool *p = new Pool();
-        p->key = r;
-        p->ctx = c;
-        p->cap = cap;
-        p->bytes = b;
-        p->slot_req.assign(cap, 0);
-        p->remaining.assign(cap, 0);
-        s->pools.push_back(p);
-        s->used += b;
-        s->peak = std::max(s->peak, s->used);
-        s->pools_created++;
-        return p;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int cap = default_cap(s, r); cap >= 1; cap /= 2) {
+            const size_t b = hos_footprint(&pp, cap);
+            if (b == 0 || s->used + b > (size_t)s->p.hbm_budget) continue;
+            hos_ctx *c = nullptr;
+            if (hos_create(&pp, cap, &c) != HOS_OK) return nullptr;
+            hos_use_graph(c, s->p.use_graph);
+            hos_set_active(c, 0);
+            Pool *p = new Pool();
+            p->key = r;
+            p->ctx = c;
+            p->cap = cap;
+            p->bytes = b;
+            p->slot_req.assign(cap, 0);
+            p->remaining.assign(cap, 0);
+            s->pools.push_back(p);
+            s->used += b;
+            s->peak = std::max(s->peak, s->used);
+            s->pools_created++;
+            return p;
+        }
+        release_idle(s, true);  // empty pools of other shapes give their memory back
     }
     return nullptr;
 }
 
-static void release_idle(hos_server *s, bool all)
+// mark the requests of a pool's active members failed (none of their steps
+// in flight counts) and empty the pool
+static void fail_pool(hos_server *s, Pool *p)
 {
-    for (size_t i = 0; i < s->pools.size();) {
-        Pool *p = s->pools[i];
-        if (p->active == 0 && (all || p->idle > 64)) {
-            hos_destroy(p->ctx);
-            s->used -= p->bytes;
-            delete p;
-            s->pools.erase(s->pools.begin() + i);
-        } else {
-            ++i;
-        }
+    std::lock_guard<std::mutex> g(s->mu);
+    for (int k = 0; k < p->active; ++k) {
+        Req &q = s->reqs[p->slot_req[k]];
+        q.state = HOS_REQ_FAILED;
+        q.t_done = s->now();
+        s->lat.push_back(q.t_done - q.t_submit);
+        s->running--;
     }
+    p->active = 0;
+    p->inflight = 0;
+    p->broken = true;
+    p->empty_since = s->now();
+}
+
+// retire the finished members of an idle pool, refilling their slots from the
+// pool's end; returns HOS_OK or the first error
+static int retire(hos_server *s, Pool *p)
+{
+    int rc = HOS_OK;
+    for (int k = p->active - 1; k >= 0; --k) {
+        if (p->remaining[k] > 0) continue;
+        const long long id = p->slot_req[k];
+        std::vector<double> u((size_t)4 * p->key.Nx * (p->key.Ny / 2 + 1));
+        double H = 0, vol = 0;
+        int e1 = hos_get_state(p->ctx, k, u.data());
+        if (e1 == HOS_OK) e1 = hos_get_energy(p->ctx, k, &H, &vol);
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            Req &q = s->reqs[id];
+            q.result.swap(u);
+            q.H = H;
+            q.vol = vol;
+            q.state = e1 == HOS_OK ? HOS_REQ_DONE : HOS_REQ_FAILED;
+            q.t_done = s->now();
+            s->lat.push_back(q.t_done - q.t_submit);
+            s->running--;
+        }
+        const int last = p->active - 1;
+        if (k != last) {
+            if (hos_copy_member(p->ctx, last, k) != HOS_OK) rc = HOS_E_DEVICE;
+            p->slot_req[k] = p->slot_req[last];
+            p->remaining[k] = p->remaining[last];
+        }
+        p->active--;
+        hos_set_active(p->ctx, p->active);
+        if (p->active == 0) p->empty_since = s->now();
+    }
+    return rc;
 }
 
 extern "C" int hos_server_step(hos_server *s, int *active)
 {
     if (!s) return HOS_E_INVALID;
     std::lock_guard<std::mutex> sg(s->step_mu);
-    const double tb = s->now();
-    // 1. admission
+    int rc = HOS_OK;
+    {
+        // wall time with steps on the device (between this iteration and the last)
+        const double tn = s->now();
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->any_inflight) s->busy += tn - s->t_last;
+        s->t_last = tn;
+    }
+    // 1. collect the pools whose steps have completed
+    for (size_t i = 0; i < s->pools.size(); ++i) {
+        Pool *p = s->pools[i];
+        if (p->inflight == 0) continue;
+        const int q = hos_query(p->ctx);
+        if (q == 1) continue;
+        if (q < 0) {
+            rc = HOS_E_DEVICE;
+            fail_pool(s, p);
+            continue;
+        }
+        const int n = hos_step_times(p->ctx, nullptr, 0);
+        std::vector<double> ms(n > 0 ? n : 0);
+        if (n > 0) hos_step_times(p->ctx, ms.data(), n);
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            s->step_ms.insert(s->step_ms.end(), ms.begin(), ms.end());
+            s->pt_steps += (double)p->active * p->key.Nx * p->key.Ny * p->inflight;
+        }
+        for (int k = 0; k < p->active; ++k) p->remaining[k] -= p->inflight;
+        p->inflight = 0;
+    }
+    // 2. retire finished members of the idle pools
+    for (Pool *p : s->pools)
+        if (p->inflight == 0 && p->active > 0 && retire(s, p) != HOS_OK) rc = HOS_E_DEVICE;
+    // 3. admission into free slots of idle pools
     {
         std::lock_guard<std::mutex> g(s->mu);
         for (auto it = s->queue.begin(); it != s->queue.end();) {
@@ -248,73 +360,31 @@ extern "C" int hos_server_step(hos_server *s, int *active)
             it = s->queue.erase(it);
         }
     }
-    // 2. one RK step of every pool, pools concurrently
-    int rc = HOS_OK, nact = 0;
-    std::vector<Pool *> stepped;
+    // 4. launch every idle pool with active members: k steps in flight on its
+    //    own stream, k = the fewest steps any member still needs (capped)
+    int nact = 0;
+    bool inflight = false;
     for (Pool *p : s->pools) {
-        if (p->active == 0) {
-            p->idle++;
-            continue;
-        }
-        // members that asked for 0 steps finish without stepping
-        bool any = false;
-        for (int k = 0; k < p->active; ++k) any |= p->remaining[k] > 0;
-        if (!any) {
-            stepped.push_back(p);
-            continue;
-        }
-        p->idle = 0;
-        if (hos_step_async(p->ctx, 1) != HOS_OK) rc = HOS_E_DEVICE;
-        stepped.push_back(p);
-    }
-    for (Pool *p : stepped) {
-        bool any = false;
-        for (int k = 0; k < p->active; ++k) any |= p->remaining[k] > 0;
-        if (!any) continue;
-        if (hos_wait(p->ctx) != HOS_OK) rc = HOS_E_DEVICE;
-        double ms = 0;
-        hos_last_step_ms(p->ctx, &ms);
-        std::lock_guard<std::mutex> g(s->mu);
-        s->step_ms.push_back(ms);
-        s->pt_steps += (double)p->active * p->key.Nx * p->key.Ny;
-        for (int k = 0; k < p->active; ++k) p->remaining[k]--;
-    }
-    // 3. retire finished members, refill their slots from the pool's end
-    for (Pool *p : stepped) {
-        for (int k = p->active - 1; k >= 0; --k) {
-            if (p->remaining[k] > 0) continue;
-            const long long id = p->slot_req[k];
-            std::vector<double> u((size_t)4 * p->key.Nx * (p->key.Ny / 2 + 1));
-            double H = 0, vol = 0;
-            const int e1 = hos_get_state(p->ctx, k, u.data());
-            hos_get_energy(p->ctx, k, &H, &vol);
-            {
-                std::lock_guard<std::mutex> g(s->mu);
-                Req &q = s->reqs[id];
-                q.result.swap(u);
-                q.H = H;
-                q.vol = vol;
-                q.state = e1 == HOS_OK ? HOS_REQ_DONE : HOS_REQ_FAILED;
-                q.t_done = s->now();
-                s->lat.push_back(q.t_done - q.t_submit);
-                s->running--;
+        if (p->inflight == 0 && p->active > 0) {
+            int k = s->p.steps_per_launch;
+            for (int m = 0; m < p->active; ++m) k = std::min(k, p->remaining[m]);
+            if (k < 1) k = 1;  // (retire leaves no member with 0 steps to go)
+            const bool injected = s->inject_fail > 0 && s->inject_fail-- > 0;
+            if (injected || hos_step_async(p->ctx, k) != HOS_OK) {
+                rc = HOS_E_DEVICE;
+                fail_pool(s, p);
+            } else {
+                p->inflight = k;
             }
-            const int last = p->active - 1;
-            if (k != last) {
-                if (hos_copy_member(p->ctx, last, k) != HOS_OK) rc = HOS_E_DEVICE;
-                p->slot_req[k] = p->slot_req[last];
-                p->remaining[k] = p->remaining[last];
-            }
-            p->active--;
-            hos_set_active(p->ctx, p->active);
         }
         nact += p->active;
+        inflight |= p->inflight > 0;
     }
     {
         std::lock_guard<std::mutex> g(s->mu);
         release_idle(s, false);
         s->peak_active = std::max(s->peak_active, nact);
-        if (!stepped.empty()) s->busy += s->now() - tb;
+        s->any_inflight = inflight;
     }
     if (active) *active = nact;
     s->cv.notify_all();
@@ -335,7 +405,11 @@ extern "C" int hos_server_start(hos_server *s)
                 if (!work) s->cv.wait_for(g, std::chrono::milliseconds(5));
                 work = !s->queue.empty() || s->running > 0;
             }
-            if (work) hos_server_step(s, nullptr);
+            if (work) {
+                hos_server_step(s, nullptr);
+                // pools with steps in flight are polled, not waited for
+                std::this_thread::sleep_for(std::chrono::microseconds(50));
+            }
         }
     });
     return HOS_OK;
@@ -367,6 +441,7 @@ extern "C" int hos_server_drain(hos_server *s, double timeout_s)
             }
         }
         hos_server_step(s, nullptr);
+        std::this_thread::sleep_for(std::chrono::microseconds(50));
     }
 }
 
@@ -387,6 +462,14 @@ extern "C" int hos_server_stats(hos_server *s, double *out, int n)
                           pct(s->qwait, 50),
                           pct(s->qwait, 99)};
     for (int i = 0; i < n && i < 12; ++i) out[i] = v[i];
+    return HOS_OK;
+}
+
+extern "C" int hos_server_fail_launches(hos_server *s, int n)
+{
+    if (!s || n < 0) return HOS_E_INVALID;
+    std::lock_guard<std::mutex> sg(s->step_mu);
+    s->inject_fail = n;
     return HOS_OK;
 }
 

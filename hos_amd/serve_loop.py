@@ -58,17 +58,16 @@ def submit(server, r, u0):
                          depth=r["depth"], g=r["g"])
 
 
-def run_alone(r, u0, precision):
-    """The same request on its own context (the reference for 'same result
-    alone and in the mix', and the FP64 path for the sampled error)."""
-    p = hos.params(Nx=r["Nx"], Ny=r["Ny"], nlevs=r["nlevs"], Lx=r["Lx"], Ly=r["Ly"], g=r["g"],
-                   depth=r["depth"], precision=precision, compensated=1)
-    s = hos.Solver(p)
-    s.set_state(u0)
-    s.use_graph(True)
-    s.step(r["nsteps"], r["dt"])
-    u = s.get_state()
-    s.close()
+def run_alone(r, u0, precision, budget=250e9):
+    """The same request alone in a server of its own (the reference for 'same
+    result alone and in the mix': its pool has the same slots, hence the same
+    launch shapes and FFT forms, as in the mix; and the FP64 path for the
+    sampled error)."""
+    srv = hos.Server(hbm_budget=budget, precision=precision)
+    rid = submit(srv, r, u0)
+    srv.drain(600.0)
+    u, _ = srv.result(rid)
+    srv.close()
     return u
 
 

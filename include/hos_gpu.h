@@ -45,7 +45,7 @@ extern "C" {
 #define HOS_FP32 1
 
 typedef struct hos_params {
-    int Nx, Ny;        /* grid (powers of two, 16..4096)                      */
+    int Nx, Ny;        /* grid (powers of two, 16..2048)                      */
     int nlevs;         /* NLevs of Csource/2dpar/euler_2d_par.h:24; M = nlevs+1 */
     double Lx, Ly;     /* domain; Kx0 = 2pi/Lx (hdf5_routines_2d_par.c:98-99) */
     double g;
@@ -122,6 +122,11 @@ int hos_set_member(hos_ctx *c, int member, double depth, double dt, double t);
 int hos_copy_member(hos_ctx *c, int src, int dst);
 int hos_step_async(hos_ctx *c, int nsteps);
 int hos_wait(hos_ctx *c);
+/* Non-blocking form of hos_wait: 1 while the steps enqueued by
+ * hos_step_async are still running; 0 once none is in flight (the steps that
+ * had been enqueued are then recorded exactly as hos_wait records them);
+ * < 0 on error. */
+int hos_query(hos_ctx *c);
 
 /* Stage-0 Hamiltonian of the last step, hosm_2d_par.c:462-523 (the
  * reference's H = mean(g eta^2 + phi eta_t), no 1/2), and volume = mean eta =

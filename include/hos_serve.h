@@ -6,14 +6,22 @@
  * Requests (grid, order, depth, dt, number of RK steps, initial state) are
  * queued by hos_server_submit.  The scheduler groups them into pools, one
  * hos_ctx per (Nx, Ny, nlevs, Lx, Ly, g), with one slot per member:
- *   - admission: a queued request enters a free slot of its pool; a pool is
- *     created when its device footprint (hos_footprint) fits the remaining
- *     HBM budget, otherwise the request waits until memory is released;
- *   - every scheduler iteration advances each pool's active members by one
- *     RK step; pools run concurrently on their own streams;
- *   - a member that has done its steps is copied out, its slot is refilled
- *     by the pool's last active member (hos_copy_member), and queued
- *     requests are admitted as slots free up.
+ *   - admission: a queued request enters a free slot of its pool while the
+ *     pool has no steps in flight; a pool is created when its device
+ *     footprint (hos_footprint) fits the remaining HBM budget (empty pools
+ *     are released first when it does not), otherwise the request waits
+ *     until memory is released;
+ *   - each pool runs on its own stream and is launched on its own: an idle
+ *     pool with active members gets k = min(steps_per_launch, the fewest
+ *     steps any of its members still needs) steps enqueued at once
+ *     (hos_step_async), and the scheduler polls the pools (hos_query)
+ *     instead of waiting for each, so a slow shape holds up no other pool;
+ *   - when a pool's steps complete, a member that has done its steps is
+ *     copied out, its slot is refilled by the pool's last active member
+ *     (hos_copy_member), queued requests are admitted into the free slots,
+ *     and the pool is launched again at once;
+ *   - a pool whose launch or steps fail marks its requests HOS_REQ_FAILED
+ *     (their steps are not counted as done) and is released.
  * Each member's arithmetic does not depend on the other members, so a
  * result is bit-identical whether its request ran alone or in a mix.
  *
@@ -35,6 +43,7 @@ typedef struct hos_server_params {
     int use_graph;      /* replay one captured step per pool (1)             */
     int max_slots;      /* members per pool; 0 = by grid size (2..64)        */
     int compensated;    /* FP32: Kahan order sums                            */
+    int steps_per_launch; /* most steps enqueued per pool launch; 0 = 8       */
 } hos_server_params;
 
 typedef struct hos_request {
@@ -67,7 +76,9 @@ int hos_server_status(hos_server *s, long long id);
  * H, volume of the last step}; then forget the request. */
 int hos_server_result(hos_server *s, long long id, double *u, double *info);
 
-/* One scheduler iteration: admit, one step of every pool, retire.  *active
+/* One scheduler iteration: collect the pools whose steps have completed,
+ * retire their finished members, admit queued requests, and launch every idle
+ * pool that has active members.  Does not wait for steps in flight.  *active
  * receives the number of running members. */
 int hos_server_step(hos_server *s, int *active);
 
@@ -84,6 +95,10 @@ int hos_server_drain(hos_server *s, double timeout_s);
  * pools created, peak bytes in pools, peak running members, queue-wait p50 s,
  * queue-wait p99 s}. */
 int hos_server_stats(hos_server *s, double *out, int n);
+
+/* Failure injection for tests of the error path: the next n pool launches
+ * fail on the host (nothing is enqueued), as a failed hos_step_async would. */
+int hos_server_fail_launches(hos_server *s, int n);
 
 #ifdef __cplusplus
 }

@@ -76,9 +76,11 @@ def test_gpu_fp64_matches_fixture(hos, name, tol64):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name,tol32", [("small", 2e-5), ("cfg1", 2e-5)])
+@pytest.mark.parametrize("name,tol32", [("small", 3e-7), ("cfg1", 5e-7)])
 def test_gpu_fp32_error_vs_fixture(hos, name, tol32):
-    """FP32 form (FP64 state) vs the FP64 oracle after all steps."""
+    """FP32 form (FP64 state) vs the FP64 oracle after all steps (measured
+    2.7e-8 / 3.0e-8 for small, 6.2e-8 / 4.4e-8 for cfg1's 200 RK4 steps on
+    eta / phi_s; the bounds leave a factor of about 8)."""
     f = load(name)
     N, nlevs, L = int(f["N"]), int(f["nlevs"]), float(f["L"])
     print(f"golden {name} fp32", flush=True)

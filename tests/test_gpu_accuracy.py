@@ -50,7 +50,9 @@ def test_fp32_1000_steps_vs_fp64(hos):
     dH64 = abs(H64[-1] - H64[0]) / abs(H64[0])
     print(f"1000 steps: eta rel L2 {e_eta:.3e}, phi {e_phi:.3e}; H drift fp32 {dH32:.3e} fp64 {dH64:.3e}; "
           f"volume fp32 {v32[-1]:.3e} fp64 {v64[-1]:.3e}")
-    assert e_eta <= 1e-5 and e_phi <= 1e-5
+    assert e_eta <= 1e-5 and e_phi <= 1e-5  # the stated accuracy target
+    # regression guard: measured 2.5e-7 (eta) and 3.1e-7 (phi_s)
+    assert e_eta <= 2e-6 and e_phi <= 2e-6
     assert abs(v32[-1] - v64[-1]) < 1e-9
     assert dH64 < 1e-3 and dH32 < 1e-3
 

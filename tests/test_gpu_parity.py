@@ -2,7 +2,8 @@
 
 Tolerances: FP64 path <= 1e-12 relative L2 for one RHS / transform (both are
 FP64 pseudo-spectral evaluations of the same expressions; they differ only in
-rounding order), FP32 path <= 2e-5 relative L2 for one RHS.
+rounding order), FP32 path <= 2e-6 relative L2 for one RHS (measured 1e-7 to
+3e-7 on the random states below; 3e-9 to 6e-9 on the smoother 2048-row ones).
 """
 import numpy as np
 import pytest
@@ -13,7 +14,7 @@ from conftest import rel_l2
 pytestmark = pytest.mark.gpu
 
 TOL64 = 1e-12
-TOL32 = 2e-5
+TOL32 = 2e-6
 
 
 def make(hos, Nx, Ny, nlevs, prec, **kw):
@@ -32,8 +33,9 @@ def test_fft_2d_vs_numpy(hos, Nx, Ny, prec):
     hu = s.fft_2d(u)
     ref = to_spec(u)
     tol = 1e-14 if prec == 0 else 3e-7
-    assert rel_l2(hu, ref) < tol
     back = s.ifft_2d(ref)
+    print(f"  r2c {rel_l2(hu, ref):.2e} c2r {rel_l2(back, u):.2e}", flush=True)
+    assert rel_l2(hu, ref) < tol
     assert rel_l2(back, u) < tol
 
 
@@ -48,6 +50,7 @@ def test_rhs_fp64_vs_oracle(hos, oracle_mod, nlevs, depth):
     u = random_state(Nx, Ny, nlevs, amp=0.08, seed=nlevs)
     ref = o.rhs_hos(u)
     got = s.rhs(u)
+    print(f"  rel L2 {rel_l2(got, ref):.2e}", flush=True)
     assert rel_l2(got, ref) < TOL64, rel_l2(got, ref)
 
 
@@ -75,6 +78,7 @@ def test_rhs_2048_rows_vs_oracle(hos, oracle_mod, Nx, Ny, prec):
     s = make(hos, Nx, Ny, nlevs, prec, Lx=Lx, Ly=Ly, depth=depth)
     u = random_state(Nx, Ny, nlevs, amp=0.05, slope=3.0, seed=11)
     err = rel_l2(s.rhs(u), o.rhs_hos(u))
+    print(f"  rel L2 {err:.2e}", flush=True)
     assert err < (TOL64 if prec == 0 else TOL32), err
 
 
@@ -95,6 +99,7 @@ def test_xpass_modes_agree(hos, oracle_mod, Nx):
     s.set_tile(2, hos.HOS_TILE_XPASS)
     assert s.get_tile(hos.HOS_TILE_XPASS) == 2
     b = s.rhs(u)
+    print(f"  rows per workgroup {rel_l2(a, ref):.2e}, row per wave {rel_l2(b, ref):.2e}", flush=True)
     assert rel_l2(a, ref) < TOL32 and rel_l2(b, ref) < TOL32, (rel_l2(a, ref), rel_l2(b, ref))
     assert rel_l2(a, b) < 1e-6
 
@@ -108,7 +113,9 @@ def test_rhs_fp32_vs_oracle(hos, oracle_mod, nlevs):
     o = oracle_mod.Oracle(Nx, Ny, nlevs, 2 * np.pi, np.pi)
     s = make(hos, Nx, Ny, nlevs, 1, Ly=np.pi)
     u = random_state(Nx, Ny, nlevs, amp=0.05, seed=7)
-    assert rel_l2(s.rhs(u), o.rhs_hos(u)) < TOL32
+    err = rel_l2(s.rhs(u), o.rhs_hos(u))
+    print(f"  rel L2 {err:.2e}", flush=True)
+    assert err < TOL32
 
 
 @pytest.mark.parametrize("nlevs", [1, 2, 3, 4])
@@ -359,10 +366,21 @@ def test_autotune_restores_state_and_tiles_agree(hos):
     s.set_state(u, 0)
     s.set_state(u, 1)
     s.set_member(0, 0.0, 0.02)
-    s.set_member(1, 0.0, 0.02)
+    s.set_member(1, 0.0, 0.035)
+    s.step_async(1)  # per-member dt
+    s.wait()
+    u1 = s.get_state(0)
+    e_before = [s.energy(i) for i in range(2)]
+    t_before = [s.get_time(i) for i in range(2)]
     tiles = s.autotune(2)
     assert all(t in (4, 8, 16) for t in tiles)
-    assert np.array_equal(s.get_state(0), u) and s.get_time(0) == 0.0
+    assert np.array_equal(s.get_state(0), u1)
+    assert [s.get_time(i) for i in range(2)] == t_before
+    assert [s.energy(i) for i in range(2)] == e_before  # H and volume restored
+    s.step_async(1)  # the per-member dt restored on the device
+    s.wait()
+    assert s.get_time(0) == pytest.approx(t_before[0] + 0.02, abs=1e-15)
+    assert s.get_time(1) == pytest.approx(t_before[1] + 0.035, abs=1e-15)
     outs = []
     for tx in (4, 8, 16):
         r = hos.Solver(p)
@@ -373,15 +391,17 @@ def test_autotune_restores_state_and_tiles_agree(hos):
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
 
 
-def test_mean_level_correction_and_volume(hos):
-    """mean_level=1 holds eta-hat(0,0) at its initial value (FP64); the
+@pytest.mark.parametrize("scheme", [1, 2, 3, 4, 5])
+def test_mean_level_correction_and_volume(hos, scheme):
+    """mean_level=1 holds eta-hat(0,0) at its initial value (FP64) with every
+    time scheme (the generic Butcher path of Dormand-Prince included); the
     reported volume is Re eta-hat(0,0) at the start of the last step."""
     Nx, Ny, nlevs = 64, 64, 3
-    print("mean level", flush=True)
+    print(f"mean level scheme={scheme}", flush=True)
     u = random_state(Nx, Ny, nlevs, amp=0.08, seed=17)
     u[0, 0, 0] = 0.0123
     for ml in (0, 1):
-        s = hos.Solver(hos.params(Nx=Nx, Ny=Ny, nlevs=nlevs, precision=1, mean_level=ml))
+        s = hos.Solver(hos.params(Nx=Nx, Ny=Ny, nlevs=nlevs, precision=1, mean_level=ml, scheme=scheme))
         s.set_state(u)
         s.step(5, 0.02)
         st = s.get_state()

@@ -325,6 +325,47 @@ def test_stage_time_not_advanced(oracle_mod):
     assert o.RampFun(t) == pytest.approx(1 - np.exp(-t * t / 0.49))
 
 
+def _linear_rhs(u, Nx, Ny, nlevs, Lx, Ly, g):
+    """Masked linear RHS (W^(1) = |k| phi-hat, -g eta-hat) from numpy's own
+    wavenumbers (signed kx; the modes where the reference table differs are
+    outside the kept band for nlevs >= 1)."""
+    kx = 2 * np.pi / Lx * np.fft.fftfreq(Nx, 1.0 / Nx)[None, :]
+    ky = 2 * np.pi / Ly * np.arange(Ny // 2 + 1)[:, None]
+    kk = np.sqrt(kx * kx + ky * ky)
+    keep = ~dealias_mask(Nx, Ny, nlevs)
+    return np.stack([kk * u[1] * keep, -g * u[0] * keep])
+
+
+@pytest.mark.parametrize("nlevs", [1, 2, 4])
+def test_ramp_closed_forms(oracle_mod, nlevs):
+    """Dommermuth ramp (hosm_2d_par.c:211-229, RampFun :526-530), pinned by
+    what the mathematics fixes: R(0) = 0 leaves exactly the masked linear RHS;
+    R(50 Tramp) = 1 in double precision gives the unramped RHS; in between the
+    RHS is lin + R(t) (unramped - lin) with R(Tramp) = 1 - 1/e and
+    R(Tramp/2) = 1 - e^(-1/4).  A sign slip on the g term, R applied to the
+    linear part, or t in place of t^2 fails one of these."""
+    Nx, Ny = 32, 16
+    Lx, Ly, g, Tr = 2 * np.pi, np.pi, 9.81, 2.5
+    u = random_state(Nx, Ny, nlevs, amp=0.08, seed=21 + nlevs, Lx=Lx, Ly=Ly)
+    plain = oracle_mod.Oracle(Nx, Ny, nlevs, Lx, Ly, g=g)
+    ramped = oracle_mod.Oracle(Nx, Ny, nlevs, Lx, Ly, g=g, rampflg=1, Tramp=Tr)
+    lin = _linear_rhs(u, Nx, Ny, nlevs, Lx, Ly, g)
+    full = plain.rhs_hos(u)
+    assert rel_l2(full, lin) > 1e-3  # the nonlinear part is visible
+    # t = 0: exactly the linear part
+    r0 = ramped.rhs_hos(u, 0.0)
+    assert np.abs(r0 - lin).max() <= 1e-15 * np.abs(lin).max()
+    # t >> Tramp: the unramped RHS
+    assert rel_l2(ramped.rhs_hos(u, 50 * Tr), full) < 1e-14
+    # in between: lin + R (full - lin)
+    for t, R in ((Tr, 1 - np.exp(-1.0)), (0.5 * Tr, 1 - np.exp(-0.25)), (1.7 * Tr, 1 - np.exp(-1.7 ** 2))):
+        want = lin + R * (full - lin)
+        assert rel_l2(ramped.rhs_hos(u, t), want) < 1e-14, t
+        assert ramped.RampFun(t) == pytest.approx(R, rel=1e-15)
+    # the ramp does not depend on the state's time, only on t (stage time t_n)
+    assert ramped.RampFun(0.0) == 0.0
+
+
 # ---------------------------------------------------------------- Hamiltonian (A10)
 def test_hamiltonian_parseval(oracle_mod):
     Nx, Ny = 32, 16
