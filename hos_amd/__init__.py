@@ -44,7 +44,7 @@ EXPORTS = [
     "hos_last_error", "hos_device_name", "hos_num_members", "hos_bytes_per_point_step", "hos_step_times",
     "hos_profile", "hos_kernel_stats", "hos_spectral_op", "hos_mult", "hos_hamiltonian",
     "hos_set_active", "hos_get_active", "hos_set_member", "hos_copy_member", "hos_step_async", "hos_wait",
-    "hos_autotune", "hos_copy_bandwidth", "hos_get_tile", "hos_query",
+    "hos_autotune", "hos_copy_bandwidth", "hos_get_tile", "hos_query", "hos_rhs_noalloc",
 ]
 HOS_TILE_C2R, HOS_TILE_R2C, HOS_TILE_PAIR, HOS_TILE_ALL, HOS_TILE_XPASS, HOS_TILE_GROUPS = 0, 1, 2, 3, 4, 5
 HOS_OP_DX, HOS_OP_DY, HOS_OP_DZ, HOS_OP_SUM, HOS_OP_DEALIAS = 1, 2, 3, 4, 5

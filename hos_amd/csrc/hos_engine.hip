@@ -1064,6 +1064,133 @@ extern "C" int hos_rhs(hos_ctx *c, const double *u, double *hrhs, double t)
     return restore_member0(c);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free RHS (hos_rhs_noalloc): the buffers a context would allocate
+// live in one file-scope device array per GPU.  In host code `scratch` names a
+// host-side stand-in; its device address is taken once per device with
+// hipGetSymbolAddress and only that address is ever passed to the GPU.
+static __device__ __align__(256) unsigned char scratch[HOS_NOALLOC_SCRATCH_BYTES];
+constexpr int NOALLOC_MAXDEV = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_dev[NOALLOC_MAXDEV];
+
+// device address of this device's scratch array (looked up on the first call
+// on the device, with it current)
+static int scratch_on_device(unsigned char **out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAXDEV) return fail(HOS_E_DEVICE, "device %d out of range", dev);
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(HOS_E_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_dev[dev] = (unsigned char *)ptr;
+    }
+    *out = g_scratch_dev[dev];
+    return HOS_OK;
+}
+
+// next 256-byte aligned region of `bytes` at offset off (advanced)
+static size_t carve(size_t &off, size_t bytes)
+{
+    const size_t o = off;
+    off = (off + bytes + 255) & ~(size_t)255;
+    return o;
+}
+
+// the tables hos_create uploads, computed on the device: kx (the same
+// expression as hos_create), twiddles exp(-2 pi i k/n) (sincospi in FP64,
+// then rounded to T), and the scalars t, dt, depth.  grid ceil(max(Nx,Ny)/NT)
+template <class T>
+__global__ __launch_bounds__(NT) void k_noalloc_tables(double *kx, cx<T> *twx, cx<T> *twy, double *scal, int nx,
+                                                       int ny, double kx0, int quirk, double t, double depth)
+{
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i < nx) {
+        kx[i] = quirk ? ((i <= nx / 2 + 1) ? i * kx0 : -(nx - i) * kx0)
+                      : ((i <= nx / 2) ? i * kx0 : -(nx - i) * kx0);
+        double sn, cs;
+        sincospi(-2.0 * i / nx, &sn, &cs);
+        twx[i] = {T(cs), T(sn)};
+    }
+    if (i < ny) {
+        double sn, cs;
+        sincospi(-2.0 * i / ny, &sn, &cs);
+        twy[i] = {T(cs), T(sn)};
+    }
+    if (i == 0) {
+        scal[0] = t;
+        scal[1] = 0.0;
+        scal[2] = depth;
+    }
+}
+
+template <class T>
+static int rhs_noalloc_impl(hos_ctx *c, unsigned char *base, const double *d_u, double *d_rhs, double t)
+{
+    const size_t rb = sizeof(T);
+    size_t off = 0;
+    const size_t oI = carve(off, (size_t)c->nI * c->fsI * 2 * rb);
+    const size_t oP = carve(off, (size_t)c->nP * c->fsP * rb);
+    const size_t oth = carve(off, (size_t)c->fs * sizeof(double));
+    const size_t okx = carve(off, (size_t)c->p.Nx * sizeof(double));
+    const size_t otx = carve(off, (size_t)c->p.Nx * sizeof(cx<T>));
+    const size_t oty = carve(off, (size_t)c->p.Ny * sizeof(cx<T>));
+    const size_t osc = carve(off, 3 * sizeof(double));
+    if (off > HOS_NOALLOC_SCRATCH_BYTES)
+        return fail(HOS_E_INVALID,
+                    "hos_rhs_noalloc: %dx%d at M=%d (%s) needs %zu bytes of scratch; the limit is %zu", c->p.Nx,
+                    c->p.Ny, c->M, c->f32 ? "FP32" : "FP64", off, (size_t)HOS_NOALLOC_SCRATCH_BYTES);
+    c->I = base + oI;
+    c->P = base + oP;
+    c->th = (double *)(base + oth);
+    c->kx = (double *)(base + okx);
+    c->twx = base + otx;
+    c->twy = base + oty;
+    double *scal = (double *)(base + osc);
+    c->t = scal;
+    c->dt = scal + 1;
+    c->depth = scal + 2;
+    const int n = std::max(c->p.Nx, c->p.Ny);
+    hipLaunchKernelGGL(k_noalloc_tables<T>, dim3((n + NT - 1) / NT), dim3(NT), 0, c->stream, c->kx, (cx<T> *)c->twx,
+                       (cx<T> *)c->twy, scal, c->p.Nx, c->p.Ny, 2 * M_PI / c->p.Lx, c->p.kx_quirk, t, c->p.depth);
+    hipLaunchKernelGGL(k_th_table, dim3((unsigned)((c->fs + NT - 1) / NT)), dim3(NT), 0, c->stream, ktab(c),
+                       c->p.Nx, 0, c->th);
+    Eng<T> e{c, 1};
+    RkArgs rk = rk_base(c);
+    rk.mode = EPI_STORE_K;
+    rk.stage = 1;  // no Hamiltonian partials
+    rk.hpart = nullptr;
+    rk.K = (cx<double> *)d_rhs;
+    int rc = e.rhs((const cx<double> *)d_u, rk);
+    if (!rc) HIP_TRY(hipGetLastError());
+    return rc;
+}
+
+extern "C" int hos_rhs_noalloc(const hos_params *p, const double *d_u, double *d_rhs, double t, void *stream)
+{
+    if (!p || !d_u || !d_rhs) return fail(HOS_E_INVALID, "null argument");
+    int rc = validate(p, 1);
+    if (rc) return rc;
+    hos_ctx c;  // host bookkeeping only; its device pointers go into the scratch array
+    c.p = *p;
+    c.nm = 1;
+    c.active = 1;
+    layout(&c);
+    const size_t n = 2 * (size_t)c.fs * sizeof(cx<double>);
+    const char *a = (const char *)d_u, *b = (const char *)d_rhs;
+    if (a < b + n && b < a + n) return fail(HOS_E_INVALID, "hos_rhs_noalloc: d_rhs overlaps d_u");
+    for (int k = HOS_TILE_C2R; k <= HOS_TILE_PAIR; ++k) c.tx[k] = default_tx(c.f32, c.logny, k);
+    c.xwave = default_xwave(&c);
+    c.stream = (hipStream_t)stream;
+    unsigned char *base = nullptr;
+    if ((rc = scratch_on_device(&base))) return rc;
+    return c.f32 ? rhs_noalloc_impl<float>(&c, base, d_u, d_rhs, t) : rhs_noalloc_impl<double>(&c, base, d_u, d_rhs, t);
+}
+
 template <class T>
 static int zvel_impl(hos_ctx *c)
 {

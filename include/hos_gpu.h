@@ -138,6 +138,28 @@ int hos_get_energy(hos_ctx *c, int member, double *H, double *volume);
  * the context's state. */
 int hos_rhs(hos_ctx *c, const double *u, double *hrhs, double t);
 
+/* Allocation-free form of hos_rhs.  It has no counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ *
+ * One RHS evaluation (rhs_hos, hosm_2d_par.c:138-235, by the same kernels as
+ * hos_rhs) of the DEVICE state d_u [2][Ny/2+1][Nx] complex double at time t
+ * into the DEVICE array d_rhs (same layout; it must not overlap d_u), for the
+ * configuration *p (scheme and mean_level are not used).  The launches are
+ * enqueued on `stream` (a hipStream_t; NULL: the null stream) and not waited
+ * for, and no host-device copy is made.  Every other buffer (x-intermediates,
+ * physical fields, FFT twiddles, wavenumber and tanh tables, the time) lives
+ * in one static device array of HOS_NOALLOC_SCRATCH_BYTES per GPU, which each
+ * GPU holds until the process ends.  A configuration that needs more returns
+ * HOS_E_INVALID (FP64 fits 256^2 at every M <= 8 and 512^2 at M <= 4;
+ * FP32 512^2 at every M <= 8).  Nothing is allocated: no hipMalloc, hipFree or
+ * allocating library routine.  All calls on one GPU share that GPU's array,
+ * which keeps the last call's contents (every kernel writes what a later
+ * kernel of the same call reads): calls in one stream are ordered, calls
+ * that overlap on one GPU (other streams, other host threads) race. */
+#define HOS_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int hos_rhs_noalloc(const hos_params *p, const double *d_u, double *d_rhs, double t, void *stream);
+
 /* W expansion of a host state (Zvel, hosm_2d_par.c:240-445): W^(M), W^(M-2),
  * (W^2)^(M), (W^2)^(M-2), each host [Ny/2+1][Nx] complex.  Any pointer may be
  * NULL. */
