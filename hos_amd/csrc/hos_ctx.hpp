@@ -52,6 +52,12 @@ struct hos_ctx {
            *mean0 = nullptr, *depth = nullptr;
     double *th = nullptr;  // [nm][Nk][Nx] tanh(|k| h) per member
     int active = 0;  // members 0..active-1 are stepped
+    // band of masked states: masked[m] = member m's state is zero on every
+    // mode the dealiasing mask removes (checked by hos_set_state); with all
+    // active members masked (and band_on), the RK steps skip those modes
+    std::vector<char> masked;
+    int band_on = 1;
+    int graph_band = -1;  // the band setting the captured graphs were built with
     std::vector<hipGraphExec_t> gexec_by_active;  // [graph_slot(active, groups, group)]
     std::vector<hipGraph_t> graph_by_active;
     bool async_pending = false;
@@ -214,7 +220,14 @@ struct Eng {
     int m0 = 0;              // first member (the launch covers m0 .. m0+nm-1)
     hipStream_t st = nullptr;  // launch stream (null: the context's)
     int inflight = 0;        // members whose launches run concurrently with this one (0: nm)
+    bool band = false;       // the stage input is a masked state: skip its masked modes
     hipStream_t S() const { return st ? st : c->stream; }
+    // spectral rows of the stage input that may be nonzero, and the column
+    // band of the loads (0: every column)
+    int nzr() const { return band ? c->my : c->Nk; }
+    int cmx() const { return band ? c->mx : 0; }
+    // fraction of the modes of a row that the column band keeps
+    double colfrac() const { return band ? (2.0 * c->mx - 1.0) / c->p.Nx : 1.0; }
     // enough spectral rows in flight to fill the GPU twice over and to outgrow
     // its caches (8 members at 1024^2; one member at 1024^2 is not)
     bool many_rows() const { return (long)(inflight > 0 ? inflight : nm) * c->Nk >= 4096; }
@@ -265,37 +278,41 @@ struct Eng {
         }
         const int ngroups = ops.ngroups;
         const int nops = (int)list.size();
+        const int nz = nzr();
         LOGN_SWITCH(c->lognx, {
             constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
-            dim3 grid((c->Nk + ROWS - 1) / ROWS, ngroups, nm);
+            dim3 grid((nz + ROWS - 1) / ROWS, ngroups, nm);
             // FP64 form: the tanh(|k| h) table is read once per group
             const double thb = sizeof(T) == 8 ? 8.0 * ngroups : 0.0;
-            Prof pf(c, S(), HOS_K_XINV, (double)nm * c->fs * (16.0 * ngroups + thb + 2.0 * sizeof(T) * nops));
+            const double rows = (double)nz * c->p.Nx;
+            Prof pf(c, S(), HOS_K_XINV, (double)nm * rows * ((16.0 * ngroups + thb) * colfrac() + 2.0 * sizeof(T) * nops));
             bool waved = false;
             if constexpr (x_wave<T>(LG)) {
                 if (c->xwave) {
-                    dim3 gw((c->Nk + XW - 1) / XW, ngroups, nm);
+                    dim3 gw((nz + XW - 1) / XW, ngroups, nm);
                     hipLaunchKernelGGL((k_xinv_w<T, LG>), gw, dim3(NT), 0, S(), Y, sY(), I(), sI(), c->fs, c->fsI, c->ip, ops,
-                                       kt(), twx());
+                                       kt(), twx(), nz, cmx());
                     waved = true;
                 }
             }
             if (!waved) {
                 hipLaunchKernelGGL((k_xinv<T, LG, ROWS>), grid, dim3(NT), 0, S(), Y, sY(), I(), sI(), c->fs, c->fsI, c->ip,
-                                   ops, kt(), twx());
+                                   ops, kt(), twx(), nz, cmx());
             }
         });
         return HOS_OK;
     }
-    int yc2r(const FieldMap &fm, int nf)
+    // nz: rows of the x-intermediates that may be nonzero (0: all)
+    int yc2r(const FieldMap &fm, int nf, int nz = 0)
     {
+        if (nz <= 0) nz = c->Nk;
         const int tx = c->tx[HOS_TILE_C2R];
         LOGN_SWITCH(c->logny, {
             TX_SWITCH(T, tx, {
                 dim3 grid(c->p.Nx / TXV, nf, nm);
-                Prof pf(c, S(), HOS_K_YC2R, (double)nm * nf * sizeof(T) * (2.0 * c->fs + c->p.Nx * (double)c->p.Ny));
+                Prof pf(c, S(), HOS_K_YC2R, (double)nm * nf * sizeof(T) * (2.0 * nz * c->p.Nx + c->p.Nx * (double)c->p.Ny));
                 hipLaunchKernelGGL((k_yc2r<T, LG, TXV>), grid, dim3(NT), 0, S(), I(), sI(), c->fsI, P(), sP(),
-                                   c->fsP, fm, (int)c->ip, twy());
+                                   c->fsP, fm, (int)c->ip, twy(), nz);
             })
         });
         return HOS_OK;
@@ -353,10 +370,19 @@ struct Eng {
         });
         return HOS_OK;
     }
+    // rows of the final x pass: the mask's my for a masked state's RK stage
+    int rhs_rows(const RkArgs &rk) const { return rk.band ? c->my : c->Nk; }
+    // stage-0 Hamiltonian partials per member that pass writes
+    int rhs_parts(const RkArgs &rk) const
+    {
+        const int rows = c->lognx >= 10 ? 1 : (1 << (10 - c->lognx));
+        return (rhs_rows(rk) + rows - 1) / rows;
+    }
     // ib < 0: the interleaved pair of yr2c_pair at field ia
     int xfwd_rhs(int ia, int ib, const cx<double> *Y, const RkArgs &rk)
     {
         const double scale = 1.0 / ((double)c->p.Nx * c->p.Ny);
+        const int nz = rhs_rows(rk);
         // FP64 [2][Nk][Nx] fields moved by the epilogue
         double f64 = 0.0;
         if (rk.mode == EPI_RK_SUBDIAG) {
@@ -368,12 +394,14 @@ struct Eng {
             f64 = 2 + 2 + (rk.hpart && rk.stage == 0 ? 2 : 0);
         else
             f64 = 2;
+        // a masked state's epilogue moves the kept modes only
+        const double kept = rk.band ? (double)c->my * (2.0 * c->mx - 1.0) : (double)c->fs;
         LOGN_SWITCH(c->lognx, {
             constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
-            dim3 grid((c->Nk + ROWS - 1) / ROWS, 1, nm);
-            Prof pf(c, S(), HOS_K_XFWD_RHS, (double)nm * c->fs * (2.0 * 2.0 * sizeof(T) + 16.0 * f64));
+            dim3 grid((nz + ROWS - 1) / ROWS, 1, nm);
+            Prof pf(c, S(), HOS_K_XFWD_RHS, (double)nm * (2.0 * 2.0 * sizeof(T) * nz * c->p.Nx + 16.0 * f64 * kept));
             hipLaunchKernelGGL((k_xfwd_rhs<T, LG, ROWS>), grid, dim3(NT), 0, S(), I(), sI(), c->fsI, c->ip, ia,
-                               ib < 0 ? ia : ib, ib < 0 ? 1 : 0, Y, sY(), scale, kt(), rk, twx());
+                               ib < 0 ? ia : ib, ib < 0 ? 1 : 0, Y, sY(), scale, kt(), rk, twx(), nz);
         });
         return HOS_OK;
     }
@@ -432,7 +460,7 @@ struct Eng {
         FieldMap fm{};
         const int n = level0(ops, fm, defer);
         int rc = xinv(Y, ops);
-        if (!rc) rc = yc2r(fm, n);
+        if (!rc) rc = yc2r(fm, n, nzr());
         for (int lev = 1; lev <= M - 1 && !rc; ++lev) {
             rc = phi_level(lev);
             const int nd = M - lev;
@@ -469,7 +497,15 @@ struct Eng {
     int yr2c_rhs()
     {
         RhsDeferred<MO> dfr;
-        for (int f = 0; f < MO + 4; ++f) dfr.slot[f] = dslot(f);
+        // the gradients (0..3) and d^M phi^(1)/dz^M (term M-1) come from the
+        // level-0 x inverse pass: a band of a masked state
+        double drows = 0.0;
+        for (int f = 0; f < MO + 4; ++f) {
+            dfr.slot[f] = dslot(f);
+            dfr.nzr[f] = (f < 4 || f == MO + 3) ? nzr() : c->Nk;
+            drows += dfr.nzr[f];
+        }
+        dfr.nzo = band ? c->my : c->Nk;
         const int nin = 1 + (MO - 1) * MO / 2;  // eta and d^j phi^(a), a + j <= M, from the physical grid
         const int tx = c->tx[HOS_TILE_PAIR];
         LOGN_SWITCH(c->logny, {
@@ -479,7 +515,7 @@ struct Eng {
                 constexpr int TXP = (TXP0 > 2 && TXP0 * (1 << LG) / V > 2 * NT_RHS) ? 2 : TXP0;
                 dim3 grid(c->p.Nx / TXP, 1, nm);
                 Prof pf(c, S(), HOS_K_YR2C_PAIR,
-                        (double)nm * sizeof(T) * (nin * (double)c->p.Nx * c->p.Ny + 2.0 * (MO + 4) * c->fs + 4.0 * c->fs));
+                        (double)nm * sizeof(T) * c->p.Nx * (nin * (double)c->p.Ny + 2.0 * drows + 4.0 * dfr.nzo));
                 hipLaunchKernelGGL((k_yr2c_rhs<T, LG, TXP, MO>), grid, dim3(NT_RHS), 0, S(), phys(), c->ix, (int)comp(),
                                    I(), sI(), c->fsI, dfr, 0, (int)c->ip, twy());
             })

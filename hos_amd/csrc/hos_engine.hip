@@ -364,6 +364,7 @@ extern "C" int hos_create(const hos_params *p, int nmembers, hos_ctx **out)
     if (rc) return bail(rc);
     c->host_t.assign(nm, 0.0);
     c->host_dt.assign(nm, 0.0);
+    c->masked.assign(nm, 1);  // zero states
     c->active = nm;
     {
         std::vector<double> dep(nm, p->depth);
@@ -437,20 +438,32 @@ static void group_range(const hos_ctx *c, int g, int ng, int *m0, int *n)
     *n = (int)((long)c->active * (g + 1) / ng) - *m0;
 }
 
+// the RK steps skip the masked modes when every active member's state is
+// masked (hos_set_state checks), with the band on and a sub-diagonal scheme
+static bool band_now(const hos_ctx *c)
+{
+    if (!c->band_on || !c->subdiag) return false;
+    for (int m = 0; m < c->active; ++m)
+        if (!c->masked[m]) return false;
+    return true;
+}
+
 // one RK step of members m0 .. m0+n-1, launched on stream st
 template <class T>
 static int launch_step(hos_ctx *c, int m0, int n, hipStream_t st)
 {
     Eng<T> e{c, n, m0, st};
     e.inflight = c->active;  // the other member groups step concurrently
-    const int nparts = x_nparts(c);
+    e.band = band_now(c);
     const long s2 = 2 * c->fs;  // member stride of the FP64 [2][Nk][Nx] arrays
-    int rc = HOS_OK;
+    int rc = HOS_OK, nparts = x_nparts(c);
     for (int s = 0; s < c->S && !rc; ++s) {
         RkArgs rk = rk_base(c);
         rk.stage = s;
         rk.c_s = c->c[s];
         rk.stream = e.many_rows();
+        rk.band = e.band;
+        nparts = e.rhs_parts(rk);  // the same at every stage
         rk.t += m0;
         rk.dt += m0;
         rk.U += m0 * s2;
@@ -583,6 +596,14 @@ static int step_enqueue(hos_ctx *c, int nsteps)
 {
     if (c->async_pending) return fail(HOS_E_STATE, "a previous hos_step_async has not been waited for");
     if (nsteps < 0) return fail(HOS_E_INVALID, "nsteps < 0");
+    {
+        // the captured graphs bake in the band's launch shapes
+        const int b = band_now(c) ? 1 : 0;
+        if (b != c->graph_band) {
+            drop_graphs(c);
+            c->graph_band = b;
+        }
+    }
     if (c->use_graph && c->active > 0) {
         int rc = build_graph(c);
         if (rc) return rc;
@@ -738,6 +759,7 @@ extern "C" int hos_copy_member(hos_ctx *c, int src, int dst)
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->host_t[dst] = c->host_t[src];
     c->host_dt[dst] = c->host_dt[src];
+    c->masked[dst] = c->masked[src];
     return HOS_OK;
 }
 
@@ -787,6 +809,13 @@ extern "C" int hos_sync(hos_ctx *c)
 extern "C" int hos_set_tile(hos_ctx *c, int kind, int tx)
 {
     if (!c) return fail(HOS_E_INVALID, "null context");
+    if (kind == HOS_TILE_BAND) {
+        if (tx < 0 || tx > 2) return fail(HOS_E_INVALID, "band setting %d: 0 default, 1 off, 2 on", tx);
+        if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
+        c->band_on = tx != 1;
+        drop_graphs(c);
+        return HOS_OK;
+    }
     if (kind == HOS_TILE_GROUPS) {
         if (tx < 0 || tx > hos_ctx::MAXG) return fail(HOS_E_INVALID, "member groups %d: 0 default, 1 .. %d", tx, hos_ctx::MAXG);
         if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
@@ -821,6 +850,7 @@ extern "C" int hos_get_tile(hos_ctx *c, int kind)
 {
     if (!c) return fail(HOS_E_INVALID, "null context");
     if (kind == HOS_TILE_XPASS) return c->xwave ? 2 : 1;
+    if (kind == HOS_TILE_BAND) return c->band_on ? 2 : 1;
     if (kind == HOS_TILE_GROUPS) return c->ngroups;
     if (kind < HOS_TILE_C2R || kind > HOS_TILE_PAIR) return fail(HOS_E_INVALID, "unknown tile kind %d", kind);
     return c->tx[kind];
@@ -830,14 +860,15 @@ extern "C" int hos_get_tile(hos_ctx *c, int kind)
 // autotuner: y tiles per kernel kind, then the x-pass mode, timed on the
 // context's own members
 static std::mutex g_tune_mu;
-static std::map<std::tuple<int, int, int, int, int, int>, std::array<int, 5>> g_tune_cache;
+static std::map<std::tuple<int, int, int, int, int, int, int>, std::array<int, 5>> g_tune_cache;
 
 extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
 {
     if (!c) return fail(HOS_E_INVALID, "null context");
     if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
     if (nsteps < 1) nsteps = 2;
-    const auto key = std::make_tuple(c->p.Nx, c->p.Ny, c->M, (int)c->f32, c->active, (int)c->subdiag);
+    const auto key = std::make_tuple(c->p.Nx, c->p.Ny, c->M, (int)c->f32, c->active, (int)c->subdiag,
+                                     (int)band_now(c));
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
         auto it = g_tune_cache.find(key);
@@ -979,6 +1010,20 @@ extern "C" int hos_set_state(hos_ctx *c, int member, const double *u)
     HIP_TRY(hipMemcpyAsync(c->Y + member * 2 * c->fs, u, n, hipMemcpyHostToDevice, c->stream));
     const double m0 = u[0];  // Re eta-hat(0,0)
     HIP_TRY(hipMemcpyAsync(c->mean0 + member, &m0, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // is the state zero on every mode the mask removes (operators_par.c:309-322)?
+    bool ok = true;
+    for (int f = 0; f < 2 && ok; ++f)
+        for (int j = 0; j < c->Nk && ok; ++j) {
+            const double *row = u + 2 * ((long)f * c->fs + (long)j * c->p.Nx);
+            for (int i = 0; i < c->p.Nx; ++i) {
+                const bool cut = (i >= c->mx && i < c->p.Nx - c->mx + 1) || j >= c->my;
+                if (cut && (row[2 * i] != 0.0 || row[2 * i + 1] != 0.0)) {
+                    ok = false;
+                    break;
+                }
+            }
+        }
+    c->masked[member] = ok ? 1 : 0;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return HOS_OK;
 }

@@ -200,13 +200,21 @@ struct XGeom {
     static constexpr int XNB = BYTES <= 8192 ? 4 : (BYTES <= 16384 ? 2 : 1);
 };
 
+// Band of a masked state (operators_par.c:309-322 leaves modes kx < mx or
+// kx > Nx-mx, ky < my only): a pass over spectral rows handles rows ky < nzr
+// (my for a masked state, Nk otherwise) and, when cmx > 0, loads only the
+// columns the mask keeps.  The modes it skips are exact zeros, which the
+// transforms would only carry along, so results are unchanged.
+__device__ __forceinline__ bool col_kept(int kx, int nx, int cmx) { return cmx == 0 || kx < cmx || kx > nx - cmx; }
+
 // x inverse pass: spectral (FP64 stage input, with multiplier) -> I.  The
 // source row is read once into registers (FP64) and every output of the
-// group is formed from it.  grid (ceil(Nk/ROWS), ngroups, members)
+// group is formed from it.  Rows ky < nzr only, columns by cmx (col_kept).
+// grid (ceil(nzr/ROWS), ngroups, members)
 template <class T, int LOGNX, int ROWS>
 __global__ __launch_bounds__(NT) void k_xinv(const cx<double> *__restrict__ Y, long sY, cx<T> *__restrict__ I,
                                              long sI, long fs, long fsI, long ip, XinvOps ops, KTab kt,
-                                             const cx<T> *__restrict__ twx)
+                                             const cx<T> *__restrict__ twx, int nzr, int cmx)
 {
     using G = XGeom<T, LOGNX>;
     static_assert(G::ROWS == ROWS, "x rows");
@@ -224,10 +232,11 @@ __global__ __launch_bounds__(NT) void k_xinv(const cx<double> *__restrict__ Y, l
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int e = threadIdx.x + q * NT;
-        const int kx = e % NX, ky = min(ky0 + e / NX, kt.nk - 1);
-        s[q] = src[(long)ky * NX + kx];
+        const int kx = e % NX, ky = min(ky0 + e / NX, nzr - 1);
+        const bool keep = col_kept(kx, NX, cmx);
+        s[q] = keep ? src[(long)ky * NX + kx] : cx<double>{0.0, 0.0};
         kxv[q] = kt.kx[kx];
-        if constexpr (sizeof(R) == 8) thd[q] = tht[(long)ky * NX + kx];
+        if constexpr (sizeof(R) == 8) thd[q] = keep ? tht[(long)ky * NX + kx] : 1.0;
         else thd[q] = 1.0;
     }
     cx<R> sr[PER];
@@ -236,7 +245,7 @@ __global__ __launch_bounds__(NT) void k_xinv(const cx<double> *__restrict__ Y, l
     for (int q = 0; q < PER; ++q) {
         const int e = threadIdx.x + q * NT;
         const int ky = ky0 + e / NX;
-        sr[q] = ky < kt.nk ? cx<R>{R(s[q].x), R(s[q].y)} : cx<R>{R(0), R(0)};
+        sr[q] = ky < nzr ? cx<R>{R(s[q].x), R(s[q].y)} : cx<R>{R(0), R(0)};
         kxr[q] = R(kxv[q]);
         kz_th<R>(kxv[q], ky * kt.ky0, h, thd[q], kz[q], th[q]);  // d^n/dz^n: k^n tanh(kh)^(n odd)
     }
@@ -279,7 +288,7 @@ __global__ __launch_bounds__(NT) void k_xinv(const cx<double> *__restrict__ Y, l
             for (int q = 0; q < PER; ++q) {
                 const int e = threadIdx.x + q * NT;
                 const int kx = e % NX, ky = ky0 + e / NX;
-                if (ky < kt.nk) dst[(long)ky * ip + kx] = buf[L::at(b * ROWS + e / NX, kx)];
+                if (ky < nzr) dst[(long)ky * ip + kx] = buf[L::at(b * ROWS + e / NX, kx)];
             }
         }
     }
@@ -298,10 +307,11 @@ struct FieldMap {
 // x-intermediate src [Nk][nx] is read transposed, rows x0+2p and x0+2p+1 are
 // the real and imaginary part of one complex inverse FFT, written to dst
 // [nx][NY].  buf holds NPR*NY complex values.  Every load of the tile is
-// issued before the first LDS write; one tile per workgroup.
+// issued before the first LDS write; one tile per workgroup.  Rows ky >= nzr
+// are zero (not read).
 template <class T, int LOGNY, int NPR>
 __device__ __forceinline__ void c2r_tile(const cx<T> *__restrict__ src, T *__restrict__ dst, int x0, int nx,
-                                         cx<T> *buf, cx<T> *twl, const cx<T> *__restrict__ twy)
+                                         cx<T> *buf, cx<T> *twl, const cx<T> *__restrict__ twy, int nzr)
 {
     constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1;
     using L = Lds<T, NY>;
@@ -318,7 +328,12 @@ __device__ __forceinline__ void c2r_tile(const cx<T> *__restrict__ src, T *__res
         const int p = (e / 16) % NPR;
         // past the end: reload the last row (in bounds, value unused)
         const int ky = min((e / (16 * NPR)) * 16 + e % 16, NK - 1);
-        load2(src + (long)ky * nx + x0 + 2 * p, A[q], B[q]);
+        if (ky < nzr) {
+            load2(src + (long)ky * nx + x0 + 2 * p, A[q], B[q]);
+        } else {
+            A[q] = {T(0), T(0)};
+            B[q] = {T(0), T(0)};
+        }
     }
 #pragma unroll
     for (int q = 0; q < PERL; ++q) {
@@ -382,14 +397,14 @@ __device__ __forceinline__ void c2r_tile(const cx<T> *__restrict__ src, T *__res
 template <class T, int LOGNY, int TX>
 __global__ __launch_bounds__(NT) void k_yc2r(const cx<T> *__restrict__ I, long sI, long fsI, T *__restrict__ P,
                                              long sP, long fsP, FieldMap fm, int nx,
-                                             const cx<T> *__restrict__ twy)
+                                             const cx<T> *__restrict__ twy, int nzr)
 {
     constexpr int NY = 1 << LOGNY, NPR = TX / 2;
     __shared__ cx<T> buf[Lds<T, NY>::size(NPR)];
     __shared__ cx<T> twl[NY / 2];
     const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TX;
     c2r_tile<T, LOGNY, NPR>(I + blockIdx.z * sI + fm.src[blockIdx.y] * fsI,
-                            P + blockIdx.z * sP + fm.dst[blockIdx.y] * fsP, x0, nx, buf, twl, twy);
+                            P + blockIdx.z * sP + fm.dst[blockIdx.y] * fsP, x0, nx, buf, twl, twy, nzr);
 }
 
 // ---------------------------------------------------------------------------
@@ -750,6 +765,8 @@ __global__ __launch_bounds__(NT) void k_yr2c_pair(PRO pro, cx<T> *__restrict__ I
 template <int M>
 struct RhsDeferred {
     int slot[M + 4];  // x-intermediate slot of eta_x, eta_y, phi_x, phi_y, then W^(M) term k = 0..M-1
+    int nzr[M + 4];   // rows that may be nonzero (a band of a masked state, or Nk)
+    int nzo;          // output rows written (my: the final x pass masks the rest)
 };
 
 // y c2r tile loads of NF fields at once (x-rows x0 .. x0+2 NPR-1 of each,
@@ -762,7 +779,8 @@ struct C2rBatch {
     static constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1;
     static constexpr int NL = ((NK + 15) / 16) * 16 * NPR, NLB = NF * NL, PERL = (NLB + NTT - 1) / NTT;
     cx<T> A[PERL], B[PERL];
-    __device__ __forceinline__ void load(const cx<T> *__restrict__ base, long fs, const int *slot, int x0, int nx)
+    __device__ __forceinline__ void load(const cx<T> *__restrict__ base, long fs, const int *slot, const int *nzr,
+                                         int x0, int nx)
     {
         const int tid = opaque_tid<NTT>();
 #pragma unroll
@@ -771,11 +789,19 @@ struct C2rBatch {
             const int i = e / NL, el = e % NL;
             const int p = (el / 16) % NPR;
             const int ky = min((el / (16 * NPR)) * 16 + el % 16, NK - 1);
-            int sl = slot[0];  // slot[i] by selects (no indexed private array)
+            int sl = slot[0], nz = nzr[0];  // slot[i] by selects (no indexed private array)
 #pragma unroll
             for (int k = 1; k < NF; ++k)
-                if (i == k) sl = slot[k];
-            load2(base + sl * fs + (long)ky * nx + x0 + 2 * p, A[q], B[q]);
+                if (i == k) {
+                    sl = slot[k];
+                    nz = nzr[k];
+                }
+            if (ky < nz) {
+                load2(base + sl * fs + (long)ky * nx + x0 + 2 * p, A[q], B[q]);
+            } else {  // a row of a masked state's band that is zero
+                A[q] = {T(0), T(0)};
+                B[q] = {T(0), T(0)};
+            }
         }
     }
     // Hermitian fill: field i of the batch to rows i NPR .. i NPR + NPR-1 of buf
@@ -859,7 +885,7 @@ __device__ __forceinline__ RhsPre<vec_t<T>, C> rhs_pre(const HosIdx &ix, const R
 // phase B of the RHS pass: N_eta, N_phi at the tasks' points from the sums of
 // rhs_pre and the deferred values, then the y r2c and the interleaved store
 template <class T, int LOGNY, int TXP, int M, bool C, int NTT, int PER>
-__device__ __forceinline__ void rhs_phase_b(cx<T> *__restrict__ out, int x0, int nx, cx<T> *buf,
+__device__ __forceinline__ void rhs_phase_b(cx<T> *__restrict__ out, int x0, int nx, int nzo, cx<T> *buf,
                                             const cx<T> *__restrict__ twy, const int *tp, const int *ty,
                                             const RhsPre<vec_t<T>, C> *pre, const vec_t<T> *ex, const vec_t<T> *ey,
                                             const vec_t<T> *px, const vec_t<T> *py, const KSum<vec_t<T>, C> *wm)
@@ -887,7 +913,7 @@ __device__ __forceinline__ void rhs_phase_b(cx<T> *__restrict__ out, int x0, int
     constexpr int NO = ((NK + 15) / 16) * 16 * TXP;
     for (int e = threadIdx.x; e < NO; e += NTT) {
         const int p = (e / 16) % TXP, ky = (e / (16 * TXP)) * 16 + e % 16;
-        if (ky >= NK) continue;
+        if (ky >= nzo) continue;  // nzo <= NK
         const cx<T> zk = buf[L::at(p, ky)];
         const cx<T> zm = buf[L::at(p, (NY - ky) & (NY - 1))];
         store2(out + 2 * ((long)ky * nx + x0 + p), cx<T>{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)},
@@ -946,9 +972,9 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
 #pragma unroll
         for (int q = 0; q < PER; ++q) ps[q].load(ph, ix, m, x0 + tp[q], ty[q]);
         if (NB > 1)
-            tb.load(Im, fsI, dfr.slot, x0, nx);
+            tb.load(Im, fsI, dfr.slot, dfr.nzr, x0, nx);
         else
-            tl.load(Im, fsI, dfr.slot, x0, nx);
+            tl.load(Im, fsI, dfr.slot, dfr.nzr, x0, nx);
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int e = threadIdx.x + q * NTT;
@@ -963,9 +989,9 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
         }
     } else {
         if (NB > 1)
-            tb.load(Im, fsI, dfr.slot, x0, nx);
+            tb.load(Im, fsI, dfr.slot, dfr.nzr, x0, nx);
         else
-            tl.load(Im, fsI, dfr.slot, x0, nx);
+            tl.load(Im, fsI, dfr.slot, dfr.nzr, x0, nx);
     }
     tws.store(twl);
     // phase A: y c2r of the deferred fields, BF at a time (BF*NPR rows share
@@ -981,9 +1007,9 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
         else
             tl.fill(buf);
         if (b + 2 < NB)
-            tb.load(Im, fsI, dfr.slot + (b + 1) * BF, x0, nx);
+            tb.load(Im, fsI, dfr.slot + (b + 1) * BF, dfr.nzr + (b + 1) * BF, x0, nx);
         else if (b + 1 < NB)
-            tl.load(Im, fsI, dfr.slot + (b + 1) * BF, x0, nx);
+            tl.load(Im, fsI, dfr.slot + (b + 1) * BF, dfr.nzr + (b + 1) * BF, x0, nx);
         if (nf == 3)
             lds_fft<T, LOGNY, 3 * NPR, NTT, true>(buf, twl);
         else if (nf == 2)
@@ -1034,7 +1060,7 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
             pre[q] = rhs_pre<T, M, C>(ix, ps, eta[q]);
         }
     }
-    rhs_phase_b<T, LOGNY, TXP, M, C, NTT, PER>(out, x0, nx, buf, twl, tp, ty, pre, ex, ey, px, py, wm);
+    rhs_phase_b<T, LOGNY, TXP, M, C, NTT, PER>(out, x0, nx, dfr.nzo, buf, twl, tp, ty, pre, ex, ey, px, py, wm);
 }
 
 // 512 threads: half the per-thread points (and accumulators) of a 256-thread
@@ -1167,6 +1193,8 @@ struct RkArgs {
     int mean_level;
     const double *mean0;  // [members]
     int stream;           // 1: last-use loads with the non-temporal hint
+    int band;             // 1: masked states (EPI_RK_SUBDIAG): the masked modes of U, A, Y are
+                          // neither read nor written (they stay zero)
 };
 
 // RHS at spectral point idx = (ky, kx) of member m from the forward-transformed
@@ -1184,6 +1212,9 @@ __device__ __forceinline__ void rk_epilogue(const RkArgs &rk, const KTab &kt, in
         k[fsd + idx] = {scale * double(nb.x), scale * double(nb.y)};
         return;
     }
+    // a masked state's masked modes stay zero: K is zero there, so u, the
+    // accumulator and the next stage input keep their zeros without a store
+    if (rk.band && masked(kx, ky, NX, kt.mx, kt.my)) return;
     const cx<double> ye = rk.stream ? load_nt(y + idx) : y[idx];
     const cx<double> yp = rk.stream ? load_nt(y + fsd + idx) : y[fsd + idx];
     const double kxv = kt.kx[kx], kyv = ky * kt.ky0, h = kt.h(m);
@@ -1248,10 +1279,13 @@ __device__ __forceinline__ void rk_epilogue(const RkArgs &rk, const KTab &kt, in
     }
 }
 
+// Rows ky < nzr only (my with RkArgs::band: the RHS is masked beyond).
+// grid (ceil(nzr/ROWS), 1, members); the stage-0 Hamiltonian partials are
+// [members][gridDim.x].
 template <class T, int LOGNX, int ROWS>
 __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, long sI, long fs, long ip, int iA,
                                                  int iB, int ilv, const cx<double> *Y, long sY, double scale,
-                                                 KTab kt, RkArgs rk, const cx<T> *__restrict__ twx)
+                                                 KTab kt, RkArgs rk, const cx<T> *__restrict__ twx, int nzr)
 {
     using G = XGeom<T, LOGNX>;
     static_assert(G::ROWS == ROWS, "x rows");
@@ -1268,7 +1302,7 @@ __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, lo
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int e = threadIdx.x + q * NT;
-            const long idx = (long)min(ky0 + e / NX, kt.nk - 1) * ip + e % NX;  // x-intermediate row
+            const long idx = (long)min(ky0 + e / NX, nzr - 1) * ip + e % NX;  // x-intermediate row
             if (ilv) {
                 // the interleaved pair of k_yr2c_pair
                 if (rk.stream)
@@ -1283,7 +1317,7 @@ __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, lo
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int e = threadIdx.x + q * NT;
-            const bool ok = ky0 + e / NX < kt.nk;
+            const bool ok = ky0 + e / NX < nzr;
             S[L::at(e / NX, e % NX)] = ok ? va[q] : cx<T>{T(0), T(0)};
             S[L::at(ROWS + e / NX, e % NX)] = ok ? vb[q] : cx<T>{T(0), T(0)};
         }
@@ -1303,7 +1337,7 @@ __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, lo
     for (int q = 0; q < PER; ++q) {
         const int e = threadIdx.x + q * NT;
         const int r = e / NX, kx = e % NX, ky = ky0 + r;
-        if (ky >= kt.nk) continue;
+        if (ky >= nzr) continue;
         const long idx = (long)ky * NX + kx;
         rk_epilogue(rk, kt, m, idx, kx, ky, NX, S[L::at(r, kx)], S[L::at(ROWS + r, kx)], scale, R, dt, y, fsd, hsum);
     }
@@ -1332,7 +1366,7 @@ constexpr int XW = NT / 64;
 template <class T, int LOGNX>
 __global__ __launch_bounds__(NT) void k_xinv_w(const cx<double> *__restrict__ Y, long sY, cx<T> *__restrict__ I,
                                                long sI, long fs, long fsI, long ip, XinvOps ops, KTab kt,
-                                               const cx<T> *__restrict__ twx)
+                                               const cx<T> *__restrict__ twx, int nzr, int cmx)
 {
     constexpr int NX = 1 << LOGNX, E = NX / 64;
     using L = Lds<T, NX>;
@@ -1340,7 +1374,7 @@ __global__ __launch_bounds__(NT) void k_xinv_w(const cx<double> *__restrict__ Y,
     __shared__ cx<T> lds[XW * L::PITCH];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int ky = blockIdx.x * XW + w;
-    if (ky >= kt.nk) return;
+    if (ky >= nzr) return;  // rows of the band only (col_kept)
     cx<T> *row = lds + w * L::PITCH;
     const int grp = blockIdx.y, m = blockIdx.z;
     const long r0 = (long)ky * NX;
@@ -1355,9 +1389,10 @@ __global__ __launch_bounds__(NT) void k_xinv_w(const cx<double> *__restrict__ Y,
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             const int kx = lane + 64 * i;
-            sd[i] = src[kx];
+            const bool keep = col_kept(kx, NX, cmx);
+            sd[i] = keep ? src[kx] : cx<double>{0.0, 0.0};
             kxv[i] = kt.kx[kx];
-            if constexpr (sizeof(R) == 8) thd[i] = tht[kx];
+            if constexpr (sizeof(R) == 8) thd[i] = keep ? tht[kx] : 1.0;
             else thd[i] = 1.0;
         }
 #pragma unroll

@@ -239,8 +239,18 @@ int hos_use_graph(hos_ctx *c, int enable);
 #define HOS_TILE_ALL 3
 #define HOS_TILE_XPASS 4 /* k_xinv, k_xfwdinv: workgroup rows or wave rows */
 #define HOS_TILE_GROUPS 5 /* member groups: 1 chain, or 2..4 on their own streams */
+/* HOS_TILE_BAND: 1 off, 2 on (0: default, on).  On: when every active
+ * member's state is zero on the modes the dealiasing mask removes
+ * (operators_par.c:309-322; hos_set_state checks), the RK steps of the
+ * schemes 2-5 skip those modes: the level-0 x inverse transforms rows
+ * ky < my only and loads the kept columns only, the y passes that read its
+ * outputs read those rows only, the RHS pass writes them only, and the RK
+ * epilogue touches the kept modes only.  The skipped modes are exact zeros,
+ * and a masked state stays masked, so results are unchanged. */
+#define HOS_TILE_BAND 6
 int hos_set_tile(hos_ctx *c, int kind, int tx);
-/* current setting of one kind (0-2: tx; HOS_TILE_XPASS: 1 or 2; HOS_TILE_GROUPS: 1..4), < 0 on error */
+/* current setting of one kind (0-2: tx; HOS_TILE_XPASS: 1 or 2; HOS_TILE_GROUPS: 1..4;
+ * HOS_TILE_BAND: 1 or 2), < 0 on error */
 int hos_get_tile(hos_ctx *c, int kind);
 
 /* Autotuner: time nsteps RK steps of the active members for every valid

@@ -427,3 +427,56 @@ def test_graph_replay_matches_eager(hos):
         a.step(4, 0.02)
         b.step(4, 0.02)
         assert np.array_equal(a.get_state(), b.get_state())
+
+
+@pytest.mark.parametrize("prec,Nx,Ny,nlevs,nm,xmode", [
+    (1, 1024, 512, 4, 4, 2),   # wave x passes, two member groups
+    (1, 512, 256, 3, 2, 1),    # two rows per x workgroup
+    (0, 256, 128, 2, 2, 1),    # FP64, four rows per x workgroup
+    (1, 128, 256, 5, 3, 1),    # eight rows per x workgroup
+])
+def test_band_of_masked_states_is_bit_identical(hos, prec, Nx, Ny, nlevs, nm, xmode):
+    """Masked states (the mask of operators_par.c:309-322 leaves their other
+    modes zero) step with the band on (the level-0 pass, the y passes that
+    read it, the RHS pass output and the RK epilogue skip the masked modes)
+    bit-identically to the band off, states and energies alike, with finite
+    depth and the ramp on; a member whose state is not masked turns the band
+    off for its batch, and it then matches a context with the band off."""
+    print(f"band prec={prec} {Nx}x{Ny} nlevs={nlevs} members={nm}", flush=True)
+    Lx, Ly = 2 * np.pi * Nx / 32, 2 * np.pi * Ny / 32
+    p = hos.params(Nx=Nx, Ny=Ny, nlevs=nlevs, Lx=Lx, Ly=Ly, precision=prec, depth=1.1, rampflg=1, Tramp=0.5)
+    states = [random_state(Nx, Ny, nlevs, amp=0.04, slope=3.0, seed=70 + i, Lx=Lx, Ly=Ly) for i in range(nm)]
+    dts = 0.02 + 0.002 * np.arange(nm)
+    out = {}
+    for band in (1, 2):
+        s = hos.Solver(p, nm)
+        s.set_tile(band, hos.HOS_TILE_BAND)
+        s.set_tile(xmode, hos.HOS_TILE_XPASS)
+        s.use_graph(band == 2)
+        for i, u in enumerate(states):
+            s.set_state(u, i)
+        s.step(4, dts)
+        out[band] = [(s.get_state(i), s.energy(i)) for i in range(nm)]
+        s.close()
+    for (u1, e1), (u2, e2) in zip(out[1], out[2]):
+        assert np.array_equal(u1, u2)
+        assert e1 == e2
+    # the masked modes stay exactly zero
+    cut = dealias_mask(Nx, Ny, nlevs)
+    assert not np.any(out[2][0][0][:, cut])
+    # an unmasked member: the batch runs without the band, as the band-off context
+    bad = states[0].copy()
+    bad[0][cut] = 1e-9
+    res = []
+    for band in (1, 2):
+        s = hos.Solver(p, nm)
+        s.set_tile(band, hos.HOS_TILE_BAND)
+        s.set_tile(xmode, hos.HOS_TILE_XPASS)
+        for i, u in enumerate(states):
+            s.set_state(bad if i == 0 else u, i)
+        s.step(2, dts)
+        res.append([s.get_state(i) for i in range(nm)])
+        s.close()
+    for a, b in zip(*res):
+        assert np.array_equal(a, b)
+    assert np.array_equal(res[1][0][0][cut], bad[0][cut])  # its masked modes kept their values
