@@ -109,26 +109,87 @@ def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_s
     return out
 
 
-def cpu_baseline(w, dt, u0):
-    """The FP64 oracle (C restatement of the reference path) on one RK4 step of
-    the same workload, one host thread."""
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """Host threads this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS when set (the share of the host that goes with one GPU)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(n, 1)
+
+
+def cpu_baseline(w, dt, u0, steps=4):
+    """The FP64 CPU restatement of the reference path (the oracle: the
+    reference's Mult structure, 3 FFTs per product) on the same workload: one
+    RK4 step on one thread, then `steps` RK4 steps with the transforms on all
+    the host threads this process may use (OpenMP), each step timed
+    (p50/p99).  The reference FFTW-MPI build needs FFTW and MPI, which this
+    image does not have."""
     import oracle
 
     o = oracle.Oracle(w["Nx"], w["Ny"], w["nlevs"], w["Lx"], w["Ly"], g=w["g"], depth=w["depth"], scheme=2)
     u = np.ascontiguousarray(u0.copy())
-    nsteps = 1
+    oracle.set_threads(1)
     t0 = time.perf_counter()
-    o.run(u, 0.0, dt, nsteps)
-    sec = time.perf_counter() - t0
+    o.step(u, 0.0, dt)
+    sec1 = time.perf_counter() - t0
+    P = oracle.set_threads(cpu_threads())
+    t = dt
+    per = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        t, _ = o.step(u, t, dt)
+        per.append(time.perf_counter() - t0)
+    oracle.set_threads(1)
+    pts = w["Nx"] * w["Ny"]
     return {
-        "value": w["Nx"] * w["Ny"] * nsteps / sec,
+        "value": pts * steps / sum(per),
         "unit": "grid-pt RK4 steps/s",
-        "cores": 1,
+        "cores": P,
         "kind": "port",
-        "sample": f"{nsteps} RK4 step of the same 1024^2 M=5 workload ({sec:.1f} s), FP64, reference "
-                  "Mult structure (3 FFTs per product), own radix-2 FFT; the reference FFTW-MPI build is not "
-                  "buildable in this image",
+        "sample": f"{steps} RK4 steps of the same 1024^2 M=5 workload on {P} threads ({sum(per):.1f} s) after "
+                  f"1 step on 1 thread ({sec1:.1f} s); FP64 CPU restatement with the reference's Mult structure "
+                  "(3 FFTs per product), own radix-2 FFT, OpenMP over the row transforms; the reference "
+                  "FFTW-MPI build is not buildable in this image (no FFTW, no MPI)",
+        "p50_ms": round(1e3 * float(np.percentile(per, 50)), 1),
+        "p99_ms": round(1e3 * float(np.percentile(per, 99)), 1),
+        "single_thread_value": pts / sec1,
+        "cpu_model": cpu_model(),
     }
+
+
+def other_workloads(serve_seconds):
+    """The ensemble, uneven-batch and serving workloads (hos_amd.serve_loop),
+    each timed on its own: 64 realisations of 512^2 at M=4 for 500 RK4 steps;
+    96 uneven requests for 300 steps; Poisson arrivals at 20 requests/s for
+    serve_seconds (the 120 s of the full serving workload is the same load
+    for longer)."""
+    from hos_amd import serve_loop as sl
+
+    out = {}
+    b = sl.batch(members=64, N=512, nlevs=3, steps=500, check=2)
+    out["ensemble_64x512_m4"] = b
+    print(f"[bench] ensemble 64 x 512^2: {b['value']:.3e} grid-pt RK4 steps/s", file=sys.stderr, flush=True)
+    e = sl.ensemble(n=96, steps=300)
+    out["uneven_96x300"] = e
+    print(f"[bench] uneven 96 x 300: {e['value']:.3e} grid-pt RK4 steps/s", file=sys.stderr, flush=True)
+    if serve_seconds > 0:
+        v = sl.serve(rate=20.0, duration=serve_seconds)
+        out["serve_poisson_20rps"] = v
+        print(f"[bench] serve {serve_seconds:.0f} s: {v['value']:.3e} grid-pt RK4 steps/s", file=sys.stderr,
+              flush=True)
+    return out
 
 
 # kernel name -> profiled kind (hos_kernel_stats): the RHS pass k_yr2c_rhs is
@@ -181,6 +242,10 @@ def main():
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--no-others", action="store_true",
+                    help="skip the ensemble, uneven-batch and serving workloads")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="arrival window of the Poisson serving workload (0: skip it)")
     ap.add_argument("--tiles", type=str, default=None,
                     help="fixed launch shapes c2r,r2c,pair[,xpass[,groups]] for the batched form instead of the autotuner "
                          "(profiler runs: no tuning trials in the kernel statistics)")
@@ -254,6 +319,10 @@ def main():
                            for n, v in ks.items() if v["launches"]},
         }
     cpu = None if args.no_cpu else cpu_baseline(w, dt, u0)
+    if cpu:
+        print(f"[bench] cpu: {cpu['value']:.3e} grid-pt RK4 steps/s on {cpu['cores']} threads", file=sys.stderr,
+              flush=True)
+    others = {} if args.no_others else other_workloads(args.serve_seconds)
 
     def line(f):
         return {"value": round(f["value"], 1), "ms_per_step": round(f["ms_per_step"], 4),
@@ -298,6 +367,7 @@ def main():
         res["steps_compared"] = args.warmup + args.steps
     res["alg_bytes_per_pt_step"] = f32["bytes_per_pt_step"]
     res["alg_GBps"] = round(f32["bytes_per_pt_step"] * f32["value"] / 1e9, 1)
+    res.update(others)
     print(json.dumps(res))
 
 

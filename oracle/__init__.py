@@ -70,6 +70,8 @@ def _load():
     lib.or_run.argtypes = [_c_p, _c_p, _d_p, ctypes.c_double, ctypes.c_int, _c_p]
     lib.or_get_tableau.argtypes = [_c_p, _c_p, _c_p, _c_p]
     lib.or_get_tableau.restype = ctypes.c_int
+    lib.or_set_threads.argtypes = [ctypes.c_int]
+    lib.or_set_threads.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -77,6 +79,12 @@ def _load():
 def _ptr(a: np.ndarray):
     assert a.flags["C_CONTIGUOUS"], "arrays passed to the oracle must be C-contiguous"
     return ctypes.c_void_p(a.ctypes.data)
+
+
+def set_threads(n: int) -> int:
+    """OpenMP threads of the oracle's transforms (n <= 0: the default);
+    returns the count in use.  Results do not depend on it."""
+    return _load().or_set_threads(n)
 
 
 def dealias_mx(n: int, nlevs: int) -> int:

@@ -119,19 +119,31 @@ static void fft1d(cplx *x, int n, int sign, const cplx *tw, cplx *scratch)
 
 /* fft_2d, fft_routines_2d_par.c:21-58: padded real u [Nx][Ny+2] -> hu
  * [Ny/2+1][Nx] (FFTW_MPI_TRANSPOSED_OUT), forward, then hu = hf/Nx/Ny. */
+/* The row and column loops below run on OpenMP threads (or_set_threads):
+ * every 1-D transform is independent and done by one thread with its own
+ * scratch, so the results do not depend on the thread count. */
 void or_fft_2d(or_ctx *C, const double *u, cplx *hu)
 {
     const int Nx = C->Nx, Ny = C->Ny, Nk = Ny / 2 + 1;
-    for (int x = 0; x < Nx; x++) {
-        for (int y = 0; y < Ny; y++) C->rowbuf[y] = u[(size_t)(Ny + 2) * x + y];
-        fft1d(C->rowbuf, Ny, -1, C->twy, C->colbuf);
-        for (int k = 0; k < Nk; k++) C->tmp[(size_t)Nx * k + x] = C->rowbuf[k];
-    }
-    const double s = 1.0 / Nx / Ny;
-    for (int k = 0; k < Nk; k++) {
-        cplx *row = &C->tmp[(size_t)Nx * k];
-        fft1d(row, Nx, -1, C->twx, C->colbuf);
-        for (int x = 0; x < Nx; x++) hu[(size_t)Nx * k + x] = row[x] * s;
+    const int nmax = Nx > Ny ? Nx : Ny;
+#pragma omp parallel
+    {
+        cplx *rowbuf = malloc(sizeof(cplx) * (size_t)nmax), *colbuf = malloc(sizeof(cplx) * (size_t)nmax);
+#pragma omp for schedule(static)
+        for (int x = 0; x < Nx; x++) {
+            for (int y = 0; y < Ny; y++) rowbuf[y] = u[(size_t)(Ny + 2) * x + y];
+            fft1d(rowbuf, Ny, -1, C->twy, colbuf);
+            for (int k = 0; k < Nk; k++) C->tmp[(size_t)Nx * k + x] = rowbuf[k];
+        }
+        const double s = 1.0 / Nx / Ny;
+#pragma omp for schedule(static)
+        for (int k = 0; k < Nk; k++) {
+            cplx *row = &C->tmp[(size_t)Nx * k];
+            fft1d(row, Nx, -1, C->twx, colbuf);
+            for (int x = 0; x < Nx; x++) hu[(size_t)Nx * k + x] = row[x] * s;
+        }
+        free(rowbuf);
+        free(colbuf);
     }
 }
 
@@ -140,13 +152,19 @@ void or_fft_2d(or_ctx *C, const double *u, cplx *hu)
 void or_ifft_2d(or_ctx *C, const cplx *hu, double *u)
 {
     const int Nx = C->Nx, Ny = C->Ny, Nk = Ny / 2 + 1;
+    const int nmax = Nx > Ny ? Nx : Ny;
+#pragma omp parallel
+    {
+    cplx *rowbuf = malloc(sizeof(cplx) * (size_t)nmax), *colbuf = malloc(sizeof(cplx) * (size_t)nmax);
+#pragma omp for schedule(static)
     for (int k = 0; k < Nk; k++) {
         cplx *row = &C->tmp[(size_t)Nx * k];
         memcpy(row, &hu[(size_t)Nx * k], sizeof(cplx) * (size_t)Nx);
-        fft1d(row, Nx, +1, C->twx, C->colbuf);
+        fft1d(row, Nx, +1, C->twx, colbuf);
     }
+#pragma omp for schedule(static)
     for (int x = 0; x < Nx; x++) {
-        cplx *r = C->rowbuf;
+        cplx *r = rowbuf;
         r[0] = creal(C->tmp[x]);
         for (int k = 1; k < Nk; k++) {
             cplx v = C->tmp[(size_t)Nx * k + x];
@@ -154,10 +172,13 @@ void or_ifft_2d(or_ctx *C, const cplx *hu, double *u)
             r[k] = v;
             r[Ny - k] = conj(v);
         }
-        fft1d(r, Ny, +1, C->twy, C->colbuf);
+        fft1d(r, Ny, +1, C->twy, colbuf);
         for (int y = 0; y < Ny; y++) u[(size_t)(Ny + 2) * x + y] = creal(r[y]);
         u[(size_t)(Ny + 2) * x + Ny] = 0.0;
         u[(size_t)(Ny + 2) * x + Ny + 1] = 0.0;
+    }
+    free(rowbuf);
+    free(colbuf);
     }
 }
 
@@ -222,6 +243,7 @@ void or_Mult(or_ctx *C, const cplx *hu1, const cplx *hu2, cplx *hprod)
 {
     or_ifft_2d(C, hu1, C->temp1);
     or_ifft_2d(C, hu2, C->temp2);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < C->Nx; i++)
         for (int j = 0; j < C->Ny; j++) {
             size_t idx = (size_t)(C->Ny + 2) * i + j;
@@ -523,6 +545,18 @@ void or_destroy(or_ctx *C)
 }
 
 ptrdiff_t or_local_N(or_ctx *C) { return C->local_N; }
+
+/* OpenMP threads of the transforms (n <= 0: the runtime's default) */
+#ifdef _OPENMP
+#include <omp.h>
+int or_set_threads(int n)
+{
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+}
+#else
+int or_set_threads(int n) { (void)n; return 1; }
+#endif
 
 /* Butcher tableau of the current scheme (for order-condition tests). */
 int or_get_tableau(or_ctx *C, double *a, double *b, double *c)

@@ -405,3 +405,20 @@ def test_stokes_wave_phase_speed(oracle_mod):
     ph = np.angle(u[0, 0, 1])
     drift = np.angle(np.exp(1j * (ph - 0.0)))
     assert abs(drift) < 0.006, drift
+
+
+def test_oracle_threads_do_not_change_results(oracle_mod):
+    """The oracle's transforms run on OpenMP threads (the multi-core CPU
+    baseline); every 1-D transform stays on one thread, so two RK4 steps give
+    the same bits on 1 and on 4 threads."""
+    Nx, Ny, nlevs = 64, 32, 3
+    u = random_state(Nx, Ny, nlevs, amp=0.06, seed=44)
+    out = []
+    for th in (1, 4):
+        oracle_mod.set_threads(th)
+        o = oracle_mod.Oracle(Nx, Ny, nlevs, 2 * np.pi, np.pi, depth=0.8)
+        v = np.ascontiguousarray(u.copy())
+        o.run(v, 0.0, 0.02, 2)
+        out.append(v)
+    oracle_mod.set_threads(1)
+    assert np.array_equal(out[0], out[1])
