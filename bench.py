@@ -69,11 +69,14 @@ def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_s
             s.set_tile(tiles[3], hos.HOS_TILE_XPASS)
         if len(tiles) > 4:
             s.set_tile(tiles[4], hos.HOS_TILE_GROUPS)
+        if len(tiles) > 5:
+            s.set_tile(tiles[5], hos.HOS_TILE_PLAN)
         tiles = tuple(tiles[:3])
     else:
         tiles = s.autotune(3) if autotune else None
     xpass = s.get_tile(hos.HOS_TILE_XPASS)
     groups = s.get_tile(hos.HOS_TILE_GROUPS)
+    plan = s.get_tile(hos.HOS_TILE_PLAN)
     s.use_graph(graph)
     s.step(warmup, dt)
     s.sync()
@@ -95,6 +98,7 @@ def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_s
         "tiles": tiles,
         "xpass": "row per wave" if xpass == 2 else "rows per workgroup",
         "groups": groups,
+        "plan": plan,
     }
     if profile_steps:
         # same workload, stream-ordered launches with a HIP event pair around
@@ -247,7 +251,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="arrival window of the Poisson serving workload (0: skip it)")
     ap.add_argument("--tiles", type=str, default=None,
-                    help="fixed launch shapes c2r,r2c,pair[,xpass[,groups]] for the batched form instead of the autotuner "
+                    help="fixed launch shapes c2r,r2c,pair[,xpass[,groups[,plan]]] for the batched form instead of the autotuner "
                          "(profiler runs: no tuning trials in the kernel statistics)")
     ap.add_argument("--calib", action="store_true", help="only the plain copy kernel (counter calibration)")
     ap.add_argument("--traffic", nargs=4, metavar=("FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
@@ -349,6 +353,7 @@ def main():
                         "members in the same launches; compared with the plain FP64 single instance",
             "members": args.members, "graph": graph, "y_tiles_c2r_r2c_pair": f32["tiles"],
             "x_inverse_passes": f32["xpass"], "member_groups": f32["groups"],
+            "rhs_fft_plan": {1: "16 8 8", 2: "8 8 4 4", 3: "4 4 4 4 4", 4: "16 16 4"}.get(f32["plan"], f32["plan"]),
             "form": "FP32 FFTs/products with Kahan order sums; FP64 state, RK combination, linear terms",
             "parallelism": "single GPU",
         },

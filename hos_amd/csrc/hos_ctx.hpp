@@ -24,6 +24,8 @@ __attribute__((visibility("hidden"))) int fail(int code, const char *fmt, ...);
 
 
 constexpr int LOGN_MIN = 4, LOGN_MAX = 11;
+// the RHS pass's alternative FFT plans are built for Ny >= 512 only
+constexpr int PLAN_LOGN_MIN = 9, NPLANS = 4;
 
 struct hos_ctx {
     hos_params p{};
@@ -31,6 +33,7 @@ struct hos_ctx {
     int nI = 0, nP = 0;
     int tx[3] = {8, 4, 4};  // y tiles: c2r, r2c, pair
     int xwave = 0;          // x inverse passes one row per wave (where available)
+    int plan = 0;           // FFT radix plan of the RHS pass (fft_stage_bits; plans 1-3 need Ny >= 512)
     // member groups: with G > 1, the active members split into G nearly equal
     // ranges that run as independent launch chains on their own streams (gst),
     // forked from and joined to the context stream each step
@@ -259,7 +262,8 @@ struct Eng {
     int xinv(const cx<double> *Y, const XinvList &list, int per = 0)
     {
         if (list.size() > MAXOPS) return fail(HOS_E_INVALID, "too many x-inverse outputs");
-        if (per <= 0) per = many_rows() ? MAXPERGRP : 3;
+        // the rows this pass has in flight: a masked state's band only
+        if (per <= 0) per = (long)(inflight > 0 ? inflight : nm) * nzr() >= 4096 ? MAXPERGRP : 3;
         XinvOps ops{};
         for (int s = 0; s < 2; ++s) {
             int k = 0;
@@ -446,9 +450,11 @@ struct Eng {
     }
 
     // y r2c of phi^(lev+1) (prologue instantiated per level) and the RHS
-    // pass (instantiated per order); defined in hos_levels.hip
+    // pass (instantiated per order; its plans 1-3 in their own unit); defined
+    // in hos_levels.hip
     int phi_level(int lev);
     int rhs_pair();
+    int rhs_pair_plan();
 
     // physical fields of the W expansion from the stage input Y; defer: the
     // fields only the RHS pass reads stay x-intermediates (dslot)
@@ -491,9 +497,9 @@ struct Eng {
         return rc;
     }
 
-    // the RHS pass of order MO (k_yr2c_rhs); TXP = half the pair tile, at most
-    // what keeps two 16-byte tasks per thread
-    template <int MO>
+    // the RHS pass of order MO (k_yr2c_rhs) with FFT plan PLAN; TXP = half the
+    // pair tile, at most what keeps two 16-byte tasks per thread
+    template <int MO, int PLAN>
     int yr2c_rhs()
     {
         RhsDeferred<MO> dfr;
@@ -516,8 +522,12 @@ struct Eng {
                 dim3 grid(c->p.Nx / TXP, 1, nm);
                 Prof pf(c, S(), HOS_K_YR2C_PAIR,
                         (double)nm * sizeof(T) * c->p.Nx * (nin * (double)c->p.Ny + 2.0 * drows + 4.0 * dfr.nzo));
-                hipLaunchKernelGGL((k_yr2c_rhs<T, LG, TXP, MO>), grid, dim3(NT_RHS), 0, S(), phys(), c->ix, (int)comp(),
-                                   I(), sI(), c->fsI, dfr, 0, (int)c->ip, twy());
+                if constexpr (PLAN != 0 && LG < PLAN_LOGN_MIN) {
+                    return fail(HOS_E_INVALID, "FFT plan %d needs Ny >= %d", PLAN, 1 << PLAN_LOGN_MIN);
+                } else {
+                    hipLaunchKernelGGL((k_yr2c_rhs<T, LG, TXP, MO, PLAN>), grid, dim3(NT_RHS), 0, S(), phys(), c->ix,
+                                       (int)comp(), I(), sI(), c->fsI, dfr, 0, (int)c->ip, twy());
+                }
             })
         });
         return HOS_OK;

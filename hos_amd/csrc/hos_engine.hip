@@ -809,6 +809,15 @@ extern "C" int hos_sync(hos_ctx *c)
 extern "C" int hos_set_tile(hos_ctx *c, int kind, int tx)
 {
     if (!c) return fail(HOS_E_INVALID, "null context");
+    if (kind == HOS_TILE_PLAN) {
+        if (tx < 0 || tx > NPLANS) return fail(HOS_E_INVALID, "FFT plan %d: 0 default, 1 .. %d", tx, NPLANS);
+        if (tx > 1 && c->logny < PLAN_LOGN_MIN)
+            return fail(HOS_E_INVALID, "FFT plans 2 .. %d need Ny >= %d", NPLANS, 1 << PLAN_LOGN_MIN);
+        if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
+        c->plan = tx == 0 ? 0 : tx - 1;
+        drop_graphs(c);
+        return HOS_OK;
+    }
     if (kind == HOS_TILE_BAND) {
         if (tx < 0 || tx > 2) return fail(HOS_E_INVALID, "band setting %d: 0 default, 1 off, 2 on", tx);
         if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
@@ -851,6 +860,7 @@ extern "C" int hos_get_tile(hos_ctx *c, int kind)
     if (!c) return fail(HOS_E_INVALID, "null context");
     if (kind == HOS_TILE_XPASS) return c->xwave ? 2 : 1;
     if (kind == HOS_TILE_BAND) return c->band_on ? 2 : 1;
+    if (kind == HOS_TILE_PLAN) return c->plan + 1;
     if (kind == HOS_TILE_GROUPS) return c->ngroups;
     if (kind < HOS_TILE_C2R || kind > HOS_TILE_PAIR) return fail(HOS_E_INVALID, "unknown tile kind %d", kind);
     return c->tx[kind];
@@ -860,7 +870,7 @@ extern "C" int hos_get_tile(hos_ctx *c, int kind)
 // autotuner: y tiles per kernel kind, then the x-pass mode, timed on the
 // context's own members
 static std::mutex g_tune_mu;
-static std::map<std::tuple<int, int, int, int, int, int, int>, std::array<int, 5>> g_tune_cache;
+static std::map<std::tuple<int, int, int, int, int, int, int>, std::array<int, 6>> g_tune_cache;
 
 extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
 {
@@ -876,6 +886,7 @@ extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
             for (int k = 0; k < 3; ++k) c->tx[k] = it->second[k];
             c->xwave = it->second[3];
             c->ngroups = it->second[4];
+            c->plan = it->second[5];
             drop_graphs(c);
             if (tiles_out)
                 for (int k = 0; k < 3; ++k) tiles_out[k] = c->tx[k];
@@ -899,7 +910,7 @@ extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
     for (int m = 0; m < c->nm; ++m) dtev[m] = c->host_dt[m] > 0 ? c->host_dt[m] : 1e-3;
     int rc = upload_dt(c, dtev.data());
     const int cand[3] = {4, 8, 16};
-    std::array<int, 5> best = {c->tx[0], c->tx[1], c->tx[2], c->xwave, c->ngroups};
+    std::array<int, 6> best = {c->tx[0], c->tx[1], c->tx[2], c->xwave, c->ngroups, c->plan};
     double best_ms = 1e30;
     const int saved_graph = c->use_graph;
     c->use_graph = 1;
@@ -930,7 +941,7 @@ extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
                 rc = trial(ms);
                 if (!rc && ms < best_ms) {
                     best_ms = ms;
-                    best = {tt[0], tt[1], tt[2], c->xwave, c->ngroups};
+                    best = {tt[0], tt[1], tt[2], c->xwave, c->ngroups, c->plan};
                 }
             }
     if (!rc && x_wave_ok(c)) {
@@ -944,11 +955,29 @@ extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
             best[3] = c->xwave;
         }
     }
-    if (!rc && c->subdiag) {
-        // the other member-group counts (each group at least two members)
-        // with the best tiles and x-pass mode
+    if (!rc && c->logny >= PLAN_LOGN_MIN) {
+        // the FFT radix plans of the RHS pass with the best tiles and x-pass mode
         for (int k = 0; k < 3; ++k) c->tx[k] = best[k];
         c->xwave = best[3];
+        const int tried = best[5];
+        for (int pl = 0; pl < NPLANS && !rc; ++pl) {
+            if (pl == tried) continue;
+            c->plan = pl;
+            double ms = 0.0;
+            rc = trial(ms);
+            if (!rc && ms < best_ms) {
+                best_ms = ms;
+                best[5] = pl;
+            }
+        }
+        c->plan = best[5];
+    }
+    if (!rc && c->subdiag) {
+        // the other member-group counts (each group at least two members)
+        // with the best tiles, x-pass mode and FFT plan
+        for (int k = 0; k < 3; ++k) c->tx[k] = best[k];
+        c->xwave = best[3];
+        c->plan = best[5];
         const int tried = best[4];
         for (int ng = 1; ng <= hos_ctx::MAXG && !rc; ++ng) {
             if (ng == tried || (ng > 1 && c->active < 2 * ng)) continue;
@@ -965,6 +994,7 @@ extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
     for (int k = 0; k < 3; ++k) c->tx[k] = best[k];
     c->xwave = best[3];
     c->ngroups = best[4];
+    c->plan = best[5];
     drop_graphs(c);
     (void)hipMemcpyAsync(c->U, saveU, st, hipMemcpyDeviceToDevice, c->stream);
     (void)hipMemcpyAsync(c->Y, saveY, st, hipMemcpyDeviceToDevice, c->stream);

@@ -49,33 +49,53 @@ struct Lds {
 // leave a single radix-2 bit behind.  Fewer stages = fewer LDS round trips.
 // Above 256 threads the bound is min(nt/2, NROWS*N/8), and the first stage,
 // which has no twiddles, needs half of it (512 threads, one to three rows of
-// 1024: 16 8 8, measured ahead of 8 8 4 4).
-__host__ __device__ constexpr int fft_stage_bits(int logn, int nrows, int s, int nt = 256)
+// 1024: 16 8 8).  That is plan 0; the autotuner may pick another family for
+// the RHS pass:
+//   plan 1: radices of at most 8 under the same rule (1024: 8 8 4 4);
+//   plan 2: radix 4 only (1024: 4 4 4 4 4);
+//   plan 3: the largest radix at every stage, no occupancy rule (16 16 4).
+__host__ __device__ constexpr int fft_stage_bits(int logn, int nrows, int s, int nt = 256, int plan = 0)
 {
+    const int cap = plan == 1 ? 3 : (plan == 2 ? 2 : 4);
     const long total = (long)nrows << logn;
     const long part = nt > 256 ? total / 8 : total / 4;
-    const long thr1 = part < nt / 2 ? part : nt / 2;
+    const long thr1 = plan == 3 ? 0 : (part < nt / 2 ? part : nt / 2);
     int rem = logn, bits = 0;
     for (int i = 0; i <= s; ++i) {
         const long thr = (i == 0 && nt > 256) ? thr1 / 2 : thr1;
         int b = rem == 1 ? 1 : 2;
-        if (4 <= rem && rem - 4 != 1 && (total >> 4) >= thr)
+        if (cap >= 4 && 4 <= rem && rem - 4 != 1 && (total >> 4) >= thr)
             b = 4;
-        else if (3 <= rem && rem - 3 != 1 && (total >> 3) >= thr)
+        else if (cap >= 3 && 3 <= rem && rem - 3 != 1 && (total >> 3) >= thr)
             b = 3;
-        if ((rem == 2 || rem == 3) && b < rem && (total >> rem) >= thr) b = rem;
+        if ((rem == 2 || rem == 3) && b < rem && rem <= cap && (total >> rem) >= thr) b = rem;
         bits = b;
         rem -= b;
         if (rem <= 0 && i < s) return 0;
     }
     return bits;
 }
-__host__ __device__ constexpr int fft_nstages(int logn, int nrows, int nt = 256)
+__host__ __device__ constexpr int fft_nstages(int logn, int nrows, int nt = 256, int plan = 0)
 {
     int n = 0;
-    while (fft_stage_bits(logn, nrows, n, nt) > 0) ++n;
+    while (fft_stage_bits(logn, nrows, n, nt, plan) > 0) ++n;
     return n;
 }
+// the stage radices of a plan as a number (16 8 8 -> 1688), for tests and notes
+__host__ __device__ constexpr long fft_plan_code(int logn, int nrows, int nt, int plan)
+{
+    long c = 0;
+    for (int s = 0; s < fft_nstages(logn, nrows, nt, plan); ++s) {
+        const int r = 1 << fft_stage_bits(logn, nrows, s, nt, plan);
+        c = c * (r >= 10 ? 100 : 10) + r;
+    }
+    return c;
+}
+static_assert(fft_plan_code(10, 3, 512, 0) == 1688, "plan 0 at 1024, three rows, 512 threads");
+static_assert(fft_plan_code(10, 3, 512, 1) == 8844, "plan 1");
+static_assert(fft_plan_code(10, 3, 512, 2) == 44444, "plan 2");
+static_assert(fft_plan_code(10, 3, 512, 3) == 16164, "plan 3");
+static_assert(fft_plan_code(10, 2, 512, 0) == 1688, "plan 0, two rows");
 
 // e^{-2 pi i m/16}, m = 0..15 (cos and sin of multiples of pi/8)
 template <class T>
@@ -240,13 +260,13 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, const cx<T> *__restrict__ 
     __syncthreads();
 }
 
-template <class T, int LOGN, int NROWS, int NT, bool INV, int S = 0, int NS = 1>
+template <class T, int LOGN, int NROWS, int NT, bool INV, int PLAN = 0, int S = 0, int NS = 1>
 __device__ __forceinline__ void fft_stages(cx<T> *buf, const cx<T> *__restrict__ tw, int tid)
 {
-    if constexpr (S < fft_nstages(LOGN, NROWS, NT)) {
-        constexpr int R = 1 << fft_stage_bits(LOGN, NROWS, S, NT);
+    if constexpr (S < fft_nstages(LOGN, NROWS, NT, PLAN)) {
+        constexpr int R = 1 << fft_stage_bits(LOGN, NROWS, S, NT, PLAN);
         fft_stage<T, R, LOGN, NROWS, NT, INV, NS>(buf, tw, tid);
-        fft_stages<T, LOGN, NROWS, NT, INV, S + 1, NS * R>(buf, tw, tid);
+        fft_stages<T, LOGN, NROWS, NT, INV, PLAN, S + 1, NS * R>(buf, tw, tid);
     }
 }
 
@@ -271,12 +291,12 @@ __device__ __forceinline__ int opaque_tid()
 // just before and read it just after.  tw: e^{-2 pi i m/N}; every plan reads
 // m < N/2 only (stage twiddle indices stay below N/R_last, or 4 N/R_last for a
 // last radix of 8 or 16)
-template <class T, int LOGN, int NROWS, int NT, bool INV>
+template <class T, int LOGN, int NROWS, int NT, bool INV, int PLAN = 0>
 __device__ __forceinline__ void lds_fft(cx<T> *buf, const cx<T> *__restrict__ tw)
 {
     const int tid = opaque_tid<NT>();  // stage addresses recomputed per call
     __syncthreads();
-    fft_stages<T, LOGN, NROWS, NT, INV>(buf, tw, tid);
+    fft_stages<T, LOGN, NROWS, NT, INV, PLAN>(buf, tw, tid);
 }
 
 // The twiddles an lds_fft reads (the first N/2 of the table) staged in LDS.

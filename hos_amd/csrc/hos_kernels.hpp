@@ -884,7 +884,7 @@ __device__ __forceinline__ RhsPre<vec_t<T>, C> rhs_pre(const HosIdx &ix, const R
 
 // phase B of the RHS pass: N_eta, N_phi at the tasks' points from the sums of
 // rhs_pre and the deferred values, then the y r2c and the interleaved store
-template <class T, int LOGNY, int TXP, int M, bool C, int NTT, int PER>
+template <class T, int LOGNY, int TXP, int M, bool C, int NTT, int PER, int PLAN>
 __device__ __forceinline__ void rhs_phase_b(cx<T> *__restrict__ out, int x0, int nx, int nzo, cx<T> *buf,
                                             const cx<T> *__restrict__ twy, const int *tp, const int *ty,
                                             const RhsPre<vec_t<T>, C> *pre, const vec_t<T> *ex, const vec_t<T> *ey,
@@ -909,7 +909,7 @@ __device__ __forceinline__ void rhs_phase_b(cx<T> *__restrict__ out, int x0, int
 #pragma unroll
         for (int v = 0; v < V; ++v) o[v] = {na[v], nb[v]};
     }
-    lds_fft<T, LOGNY, TXP, NTT, false>(buf, twy);
+    lds_fft<T, LOGNY, TXP, NTT, false, PLAN>(buf, twy);
     constexpr int NO = ((NK + 15) / 16) * 16 * TXP;
     for (int e = threadIdx.x; e < NO; e += NTT) {
         const int p = (e / 16) % TXP, ky = (e / (16 * TXP)) * 16 + e % 16;
@@ -931,7 +931,7 @@ constexpr bool rhs_early()
     return sizeof(T) == 4 && (M - 1) * M / 2 <= 10;
 }
 
-template <class T, int LOGNY, int TXP, int M, bool C, int NTT>
+template <class T, int LOGNY, int TXP, int M, bool C, int NTT, int PLAN>
 __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, const cx<T> *__restrict__ Im,
                                          long fsI, const RhsDeferred<M> &dfr, cx<T> *__restrict__ out, int m,
                                          int x0, int nx, cx<T> *buf, vec_t<T> *pbuf, cx<T> *twl,
@@ -1011,11 +1011,11 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
         else if (b + 1 < NB)
             tl.load(Im, fsI, dfr.slot + (b + 1) * BF, dfr.nzr + (b + 1) * BF, x0, nx);
         if (nf == 3)
-            lds_fft<T, LOGNY, 3 * NPR, NTT, true>(buf, twl);
+            lds_fft<T, LOGNY, 3 * NPR, NTT, true, PLAN>(buf, twl);
         else if (nf == 2)
-            lds_fft<T, LOGNY, 2 * NPR, NTT, true>(buf, twl);
+            lds_fft<T, LOGNY, 2 * NPR, NTT, true, PLAN>(buf, twl);
         else
-            lds_fft<T, LOGNY, NPR, NTT, true>(buf, twl);
+            lds_fft<T, LOGNY, NPR, NTT, true, PLAN>(buf, twl);
 #pragma unroll
         for (int i = 0; i < BF; ++i) {
             if (i >= nf) break;
@@ -1060,14 +1060,15 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
             pre[q] = rhs_pre<T, M, C>(ix, ps, eta[q]);
         }
     }
-    rhs_phase_b<T, LOGNY, TXP, M, C, NTT, PER>(out, x0, nx, dfr.nzo, buf, twl, tp, ty, pre, ex, ey, px, py, wm);
+    rhs_phase_b<T, LOGNY, TXP, M, C, NTT, PER, PLAN>(out, x0, nx, dfr.nzo, buf, twl, tp, ty, pre, ex, ey, px, py, wm);
 }
 
 // 512 threads: half the per-thread points (and accumulators) of a 256-thread
 // workgroup on the same tile
 constexpr int NT_RHS = 512;
 
-template <class T, int LOGNY, int TXP, int M>
+// PLAN: the radix family of its row FFTs (fft_stage_bits; the autotuner's choice)
+template <class T, int LOGNY, int TXP, int M, int PLAN>
 __global__ __launch_bounds__(NT_RHS) void k_yr2c_rhs(Phys<T> ph, HosIdx ix, int comp, cx<T> *__restrict__ I, long sI,
                                                      long fsI, RhsDeferred<M> dfr, int dst, int nx,
                                                      const cx<T> *__restrict__ twy)
@@ -1085,9 +1086,9 @@ __global__ __launch_bounds__(NT_RHS) void k_yr2c_rhs(Phys<T> ph, HosIdx ix, int 
     const cx<T> *Im = I + m * sI;
     cx<T> *out = I + m * sI + dst * fsI;
     if (comp)
-        rhs_tile<T, LOGNY, TXP, M, true, NT_RHS>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, pbuf, twl, twy);
+        rhs_tile<T, LOGNY, TXP, M, true, NT_RHS, PLAN>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, pbuf, twl, twy);
     else
-        rhs_tile<T, LOGNY, TXP, M, false, NT_RHS>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, pbuf, twl, twy);
+        rhs_tile<T, LOGNY, TXP, M, false, NT_RHS, PLAN>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, pbuf, twl, twy);
 }
 
 // ---------------------------------------------------------------------------

@@ -248,18 +248,26 @@ int hos_use_graph(hos_ctx *c, int enable);
  * epilogue touches the kept modes only.  The skipped modes are exact zeros,
  * and a masked state stays masked, so results are unchanged. */
 #define HOS_TILE_BAND 6
+/* HOS_TILE_PLAN: the radix plan of the RHS pass's row FFTs (0: default, 1):
+ * 1 the built-in rule (1024 points: 16 8 8), 2 radices of at most 8
+ * (8 8 4 4), 3 radix 4 (4 4 4 4 4), 4 the largest radix at every stage
+ * (16 16 4); 2-4 need Ny >= 512.  Plans round differently; hos_autotune
+ * tries them all. */
+#define HOS_TILE_PLAN 7
 int hos_set_tile(hos_ctx *c, int kind, int tx);
 /* current setting of one kind (0-2: tx; HOS_TILE_XPASS: 1 or 2; HOS_TILE_GROUPS: 1..4;
- * HOS_TILE_BAND: 1 or 2), < 0 on error */
+ * HOS_TILE_BAND: 1 or 2; HOS_TILE_PLAN: 1..4), < 0 on error */
 int hos_get_tile(hos_ctx *c, int kind);
 
 /* Autotuner: time nsteps RK steps of the active members for every valid
- * (c2r, r2c, pair) tile combination, then the other x-pass mode and the
- * other member-group count with the best tiles; keep the fastest and store it in an in-process cache keyed by
- * (Nx, Ny, M, precision, active members, scheme kind); later calls with the
- * same key apply the cached choice at once.  The members' state and time are
- * restored afterwards.  tiles_out[3] (may be NULL) receives the y tiles;
- * hos_get_tile(c, HOS_TILE_XPASS) the x-pass mode. */
+ * (c2r, r2c, pair) tile combination, then with the best tiles the other
+ * x-pass mode, then every FFT radix plan of the RHS pass (Ny >= 512), then
+ * the other member-group counts; keep the fastest and store it in an
+ * in-process cache keyed by (Nx, Ny, M, precision, active members, scheme
+ * kind, band); later calls with the same key apply the cached choice at once.
+ * The members' state, time, dt and energies are restored afterwards.
+ * tiles_out[3] (may be NULL) receives the y tiles; hos_get_tile gives the
+ * other choices. */
 int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out);
 
 const char *hos_last_error(void);

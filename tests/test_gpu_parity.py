@@ -480,3 +480,30 @@ def test_band_of_masked_states_is_bit_identical(hos, prec, Nx, Ny, nlevs, nm, xm
     for a, b in zip(*res):
         assert np.array_equal(a, b)
     assert np.array_equal(res[1][0][0][cut], bad[0][cut])  # its masked modes kept their values
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_rhs_fft_plans_vs_oracle(hos, oracle_mod, prec):
+    """Every FFT radix plan of the RHS pass (HOS_TILE_PLAN 1-4: the built-in
+    rule, radices of at most 8, radix 4, the largest radix at every stage)
+    gives the oracle's RHS within the tolerance of its precision; the plans
+    other than the built-in one are refused below Ny = 512."""
+    Nx, Ny, nlevs, depth = 64, 512, 4, 0.9
+    Lx, Ly = 2 * np.pi * 2, 2 * np.pi * 16
+    print(f"rhs fft plans prec={prec}", flush=True)
+    o = oracle_mod.Oracle(Nx, Ny, nlevs, Lx, Ly, depth=depth)
+    s = make(hos, Nx, Ny, nlevs, prec, Lx=Lx, Ly=Ly, depth=depth)
+    u = random_state(Nx, Ny, nlevs, amp=0.05, slope=3.0, seed=19, Lx=Lx, Ly=Ly)
+    ref = o.rhs_hos(u)
+    for plan in (1, 2, 3, 4):
+        s.set_tile(plan, hos.HOS_TILE_PLAN)
+        assert s.get_tile(hos.HOS_TILE_PLAN) == plan
+        err = rel_l2(s.rhs(u), ref)
+        print(f"  plan {plan}: rel L2 {err:.2e}", flush=True)
+        assert err < (TOL64 if prec == 0 else TOL32), (plan, err)
+    s.close()
+    small = make(hos, 64, 256, 2, prec)
+    with pytest.raises(hos.HosError):
+        small.set_tile(2, hos.HOS_TILE_PLAN)
+    small.set_tile(1, hos.HOS_TILE_PLAN)
+    small.close()
