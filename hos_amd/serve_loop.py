@@ -94,27 +94,55 @@ def sampled_error(reqs, results, frac, seed=0):
     return ids, errs
 
 
-def ensemble(n=96, steps=300, budget=250e9, check_frac=0.05):
-    reqs = {}
-    for i in range(n):
-        r = mix_request(i, hmin=steps, hmax=steps)
-        reqs[i] = (r, initial_state(r))
+def shape_key(r):
+    return (r["Nx"], r["Ny"], r["nlevs"])
+
+
+def run_batch(reqs, budget, label):
+    """Submit every request of reqs {i: (r, u0)} at once and drain; returns
+    (wall seconds, results, stats)."""
     srv = hos.Server(hbm_budget=budget, precision=hos.HOS_FP32)
     ids = {}
     t0 = time.perf_counter()
     for i, (r, u0) in reqs.items():
         ids[i] = submit(srv, r, u0)
-    drain_reporting(srv, n, "ensemble")
+    drain_reporting(srv, len(reqs), label)
     wall = time.perf_counter() - t0
     results = {i: srv.result(ids[i]) for i in reqs}
     st = srv.stats()
     srv.close()
+    return wall, results, st
+
+
+def ensemble(n=96, steps=300, budget=250e9, check_frac=0.05, by_shape=True):
+    """The uneven batch: n requests of the mix, all submitted at once.  With
+    by_shape, the same requests also run one shape at a time (each shape's
+    requests alone in the server, shapes one after another): the sum of those
+    walls over the mixed wall is how much the mix keeps of single-shape
+    batches (1.0: nothing lost to mixing)."""
+    reqs = {}
+    for i in range(n):
+        r = mix_request(i, hmin=steps, hmax=steps)
+        reqs[i] = (r, initial_state(r))
+    wall, results, st = run_batch(reqs, budget, "ensemble")
     pts = sum(r["Nx"] * r["Ny"] * r["nsteps"] for r, _ in reqs.values())
+    out = dict(mode="ensemble", requests=n, steps=steps, wall_s=wall, value=pts / wall,
+               unit="grid-pt RK4 steps/s", step_p50_ms=st["step_p50_ms"], step_p99_ms=st["step_p99_ms"],
+               pools=st["pools"], peak_bytes=st["peak_bytes"])
+    if by_shape:
+        shapes = sorted({shape_key(r) for r, _ in reqs.values()})
+        walls = []
+        for k in shapes:
+            sub = {i: v for i, v in reqs.items() if shape_key(v[0]) == k}
+            w, res, _ = run_batch(sub, budget, f"shape {k}")
+            walls.append(w)
+            for i in sub:  # the same bits alone in its shape's batch and in the mix
+                assert np.array_equal(res[i][0], results[i][0]), i
+        out.update(by_shape_wall_s=sum(walls), shapes=len(shapes),
+                   mix_vs_by_shape=sum(walls) / wall)
     sid, errs = sampled_error(reqs, results, check_frac)
-    return dict(mode="ensemble", requests=n, steps=steps, wall_s=wall, value=pts / wall,
-                unit="grid-pt RK4 steps/s", step_p50_ms=st["step_p50_ms"], step_p99_ms=st["step_p99_ms"],
-                pools=st["pools"], peak_bytes=st["peak_bytes"], sampled=len(sid),
-                eta_rel_l2_vs_fp64_max=max(errs))
+    out.update(sampled=len(sid), eta_rel_l2_vs_fp64_max=max(errs))
+    return out
 
 
 def serve(rate=20.0, duration=120.0, hmin=200, hmax=2000, budget=250e9, check_frac=0.05, seed=0):
