@@ -1,7 +1,10 @@
 /*
  * hos_ref.c -- the reference's call boundary (include/hos_ref.h) on top of the
  * native GPU API (include/hos_gpu.h).  Host C; every numerical operation runs
- * on the GPU through libhos_gpu.so.
+ * on the GPU through libhos_gpu.so.  Built twice: with hos_real = double
+ * (libhos_ref.so, the reference's USE_DOUBLES build) and, with
+ * -DHOS_REF_FLOAT, hos_real = float (libhos_ref_f32.so), where the arrays
+ * are converted to and from FP64 at the boundary (din / dout / dback).
  */
 #include "../../include/hos_ref.h"
 
@@ -14,12 +17,16 @@
 
 /* euler_2d_par.h:40-71 and fft_routines_2d_par.h:28-33 */
 int Nx, Ny, runsubid, saveflg, rampflg;
-double Lx, Ly, Kx0, Ky0, g, T, dtsave, Tramp;
+hos_real Lx, Ly, Kx0, Ky0, g, T, dtsave, Tramp;
 ptrdiff_t alloc_local, local_Nx, local_Nyhpo, local_N, local_0_start, local_1_start;
 int mpi_rank = 0, mpi_size = 1;
 
 int hos_nlevs = 2;
+#ifdef HOS_REF_FLOAT
+int hos_precision = HOS_FP32; /* the float build computes in the FP32 form */
+#else
 int hos_precision = HOS_FP64;
+#endif
 double hos_depth = 0.0;
 int hos_kx_quirk = 1;
 int hos_runtime_log = 1;
@@ -34,6 +41,50 @@ static void die(const char *where)
 {
     fprintf(stderr, "hos: %s failed: %s\n", where, hos_last_error());
     exit(EXIT_FAILURE);
+}
+
+/* FP64 views of boundary arrays for the native API.  hos_real = double: the
+ * array itself.  hos_real = float: a copy in one of the boundary's FP64
+ * buffers (slot k), converted back by dback for outputs. */
+#define NSLOT 6
+static double *slot_buf[NSLOT];
+static size_t slot_len[NSLOT];
+
+static double *slot(int k, size_t n)
+{
+    if (slot_len[k] < n) {
+        free(slot_buf[k]);
+        slot_buf[k] = malloc(n * sizeof(double));
+        if (!slot_buf[k]) {
+            fprintf(stderr, "hos: out of host memory\n");
+            exit(EXIT_FAILURE);
+        }
+        slot_len[k] = n;
+    }
+    return slot_buf[k];
+}
+
+/* n = number of hos_real values */
+static const double *din(int k, const void *a, size_t n)
+{
+    if (!a) return NULL;
+    if (sizeof(hos_real) == sizeof(double)) return (const double *)a;
+    double *d = slot(k, n);
+    const hos_real *f = (const hos_real *)a;
+    for (size_t i = 0; i < n; i++) d[i] = f[i];
+    return d;
+}
+static double *dout(int k, void *a, size_t n)
+{
+    if (!a) return NULL;
+    if (sizeof(hos_real) == sizeof(double)) return (double *)a;
+    return slot(k, n);
+}
+static void dback(void *a, const double *d, size_t n)
+{
+    if (!a || sizeof(hos_real) == sizeof(double)) return;
+    hos_real *f = (hos_real *)a;
+    for (size_t i = 0; i < n; i++) f[i] = (hos_real)d[i];
 }
 #define CHECK(call)                                                                                \
     do {                                                                                           \
@@ -102,7 +153,10 @@ static int in_state(const void *p)
 void hos_ref_sync(void)
 {
     if (ctx && reg_u && host_stale) {
-        CHECK(hos_get_state(ctx, 0, (double *)reg_u));
+        const size_t n = 4 * (size_t)alloc_local;
+        double *d = dout(0, reg_u, n);
+        CHECK(hos_get_state(ctx, 0, d));
+        dback(reg_u, d, n);
         host_stale = 0;
     }
 }
@@ -133,13 +187,13 @@ void Setup_TimeScheme(int flg)
     scheme_flg = flg;
 }
 
-void sol_update_RK(hos_cplx *u, double *t, double dt, char *dtflag)
+void sol_update_RK(hos_cplx *u, hos_real *t, hos_real dt, char *dtflag)
 {
     (void)dtflag;
     ensure_ctx();
     if (u != reg_u || dev_stale) {
         if (reg_u && u != reg_u) hos_ref_sync();
-        CHECK(hos_set_state(ctx, 0, (const double *)u));
+        CHECK(hos_set_state(ctx, 0, din(0, u, 4 * (size_t)alloc_local)));
         reg_u = u;
         dev_stale = host_stale = 0;
     }
@@ -179,57 +233,76 @@ void rhs_hos_clean(void)
     teardown();
 }
 
-void rhs_hos(hos_cplx *hrhs, hos_cplx *hu, double t)
+void rhs_hos(hos_cplx *hrhs, hos_cplx *hu, hos_real t)
 {
+    const size_t n = 4 * (size_t)alloc_local;
     ensure_ctx();
     touch(hu);
     touch(hrhs);
-    CHECK(hos_rhs(ctx, (const double *)hu, (double *)hrhs, t));
+    const double *u = din(0, hu, n);
+    double *r = dout(1, hrhs, n);
+    CHECK(hos_rhs(ctx, u, r, t));
+    dback(hrhs, r, n);
     wrote(hrhs);
 }
 
-void Zvel(hos_cplx *hu, hos_cplx *hZvelM, hos_cplx *hZvelM2, hos_cplx *hZvel2M, hos_cplx *hZvel2M2, double t)
+void Zvel(hos_cplx *hu, hos_cplx *hZvelM, hos_cplx *hZvelM2, hos_cplx *hZvel2M, hos_cplx *hZvel2M2, hos_real t)
 {
     (void)t;
     hos_cplx *outs[4] = {hZvelM, hZvelM2, hZvel2M, hZvel2M2};
+    const size_t n = 2 * (size_t)alloc_local;
+    double *d[4];
     ensure_ctx();
     touch(hu);
     for (int q = 0; q < 4; q++) touch(outs[q]);
-    CHECK(hos_zvel(ctx, (const double *)hu, (double *)hZvelM, (double *)hZvelM2, (double *)hZvel2M,
-                   (double *)hZvel2M2));
-    for (int q = 0; q < 4; q++) wrote(outs[q]);
+    const double *u = din(0, hu, 2 * n);
+    for (int q = 0; q < 4; q++) d[q] = dout(1 + q, outs[q], n);
+    CHECK(hos_zvel(ctx, u, d[0], d[1], d[2], d[3]));
+    for (int q = 0; q < 4; q++) {
+        dback(outs[q], d[q], n);
+        wrote(outs[q]);
+    }
 }
 
 void ZvelLinear(const hos_cplx *hu, hos_cplx *out)
 {
+    const size_t n = 2 * (size_t)alloc_local;
     ensure_ctx();
     touch(hu);
     touch(out);
-    CHECK(hos_spectral_op(ctx, HOS_OP_DZ, (const double *)(hu + alloc_local), NULL, (double *)out));
+    const double *a = din(0, hu + alloc_local, n);
+    double *o = dout(1, out, n);
+    CHECK(hos_spectral_op(ctx, HOS_OP_DZ, a, NULL, o));
+    dback(out, o, n);
     wrote(out);
 }
 
-double Hamiltonian(const hos_cplx *heta, const hos_cplx *heta_t, const hos_cplx *hphi)
+hos_real Hamiltonian(const hos_cplx *heta, const hos_cplx *heta_t, const hos_cplx *hphi)
 {
+    const size_t n = 2 * (size_t)alloc_local;
     double H = 0.0;
     ensure_ctx();
     touch(heta);
     touch(heta_t);
     touch(hphi);
-    CHECK(hos_hamiltonian(ctx, (const double *)heta, (const double *)heta_t, (const double *)hphi, &H));
-    return H;
+    CHECK(hos_hamiltonian(ctx, din(0, heta, n), din(1, heta_t, n), din(2, hphi, n), &H));
+    return (hos_real)H;
 }
 
-double RampFun(const double t) { return 1 - exp(-t * t / Tramp / Tramp); }
+hos_real RampFun(const hos_real t) { return 1 - exp(-t * t / Tramp / Tramp); }
 
 /* ---------------------------------------------------------------- L2 */
 static void unary(int op, const hos_cplx *a, const hos_cplx *b, hos_cplx *out, const char *name)
 {
+    const size_t n = 2 * (size_t)alloc_local;
     ensure_ctx();
     touch(a);
     touch(b);
     touch(out);
-    if (hos_spectral_op(ctx, op, (const double *)a, (const double *)b, (double *)out) != HOS_OK) die(name);
+    const double *da = din(0, a, n), *db = din(1, b, n);
+    double *o = dout(2, out, n);
+    if (hos_spectral_op(ctx, op, da, db, o) != HOS_OK) die(name);
+    dback(out, o, n);
     wrote(out);
 }
 
@@ -241,28 +314,40 @@ void Dealias(hos_cplx *hu) { unary(HOS_OP_DEALIAS, hu, NULL, hu, "Dealias"); }
 
 void Mult(hos_cplx *hu1, hos_cplx *hu2, hos_cplx *hprod)
 {
+    const size_t n = 2 * (size_t)alloc_local;
     ensure_ctx();
     touch(hu1);
     touch(hu2);
     touch(hprod);
-    CHECK(hos_mult(ctx, (const double *)hu1, (const double *)hu2, (double *)hprod));
+    const double *a = din(0, hu1, n), *b = din(1, hu2, n);
+    double *o = dout(2, hprod, n);
+    CHECK(hos_mult(ctx, a, b, o));
+    dback(hprod, o, n);
     wrote(hprod);
 }
 
 /* ---------------------------------------------------------------- L1 */
-void fft_2d(double *u, hos_cplx *hu, hos_plan plan)
+void fft_2d(hos_real *u, hos_cplx *hu, hos_plan plan)
 {
     (void)plan;
+    const size_t nr = (size_t)Nx * (Ny + 2), n = 2 * (size_t)alloc_local;
     ensure_ctx();
     touch(hu);
-    CHECK(hos_fft_2d(ctx, u, (double *)hu));
+    const double *a = din(0, u, nr);
+    double *o = dout(1, hu, n);
+    CHECK(hos_fft_2d(ctx, a, o));
+    dback(hu, o, n);
     wrote(hu);
 }
 
-void ifft_2d(hos_cplx *hu, double *u, hos_plan plan)
+void ifft_2d(hos_cplx *hu, hos_real *u, hos_plan plan)
 {
     (void)plan;
+    const size_t nr = (size_t)Nx * (Ny + 2), n = 2 * (size_t)alloc_local;
     ensure_ctx();
     touch(hu);
-    CHECK(hos_ifft_2d(ctx, (const double *)hu, u));
+    const double *a = din(0, hu, n);
+    double *o = dout(1, u, nr);
+    CHECK(hos_ifft_2d(ctx, a, o));
+    dback(u, o, nr);
 }

@@ -5,9 +5,13 @@
  * against libhos_ref.so (and libhos_gpu.so, with -fcommon) instead of
  * time_schemes_par.o, hosm_2d_par.o, operators_par.o and fft_routines_2d_par.o.
  *
- * Types: hos_cplx is double[2], the same type as fftw_complex, so pointers
- * declared fftw_complex* pass unchanged.  hos_plan stands in for fftw_plan; the
- * FFTW-MPI plans are replaced outright and the argument is ignored.
+ * Types: hos_cplx is hos_real[2]: double[2], the same type as fftw_complex,
+ * or float[2] = fftwf_complex in the float build, so pointers declared
+ * fftw_complex* (fftwf_complex*) pass unchanged.  hos_plan stands in for
+ * fftw_plan; the FFTW-MPI plans are replaced outright and the argument is
+ * ignored.  The float build converts its arrays to FP64 at the boundary,
+ * keeps the GPU state in FP64 between steps, and computes in the FP32 form
+ * (hos_precision defaults to HOS_FP32 there).
  *
  * Globals: the parameters the reference keeps in header-defined globals
  * (euler_2d_par.h:40-71, fft_routines_2d_par.h:28-33) are defined here; a
@@ -34,12 +38,22 @@
 extern "C" {
 #endif
 
-typedef double hos_cplx[2];
+/* TYPE_REAL of the reference (euler_2d_par.h:33-37): double in its
+ * USE_DOUBLES build (libhos_ref.so), float in the other one (define
+ * HOS_REF_FLOAT and link libhos_ref_f32.so: the same names over
+ * fftwf_complex / float, hosm_2d_par.h:31-38, operators_par.h:24-30,
+ * fft_routines_2d_par.h:45-47, time_schemes_par.h:22-24). */
+#ifdef HOS_REF_FLOAT
+typedef float hos_real;
+#else
+typedef double hos_real;
+#endif
+typedef hos_real hos_cplx[2];
 typedef void *hos_plan;
 
 /* euler_2d_par.h:40-71 */
 extern int Nx, Ny, runsubid, saveflg, rampflg;
-extern double Lx, Ly, Kx0, Ky0, g, T, dtsave, Tramp;
+extern hos_real Lx, Ly, Kx0, Ky0, g, T, dtsave, Tramp;
 /* fft_routines_2d_par.h:28-33 (one GPU: the whole grid is local) */
 extern ptrdiff_t alloc_local, local_Nx, local_Nyhpo, local_N, local_0_start, local_1_start;
 extern int mpi_rank, mpi_size;
@@ -59,16 +73,16 @@ ptrdiff_t hos_local_size_2d_transposed(ptrdiff_t n0, ptrdiff_t n1, ptrdiff_t *lo
 
 /* time_schemes_par.h:20-26 */
 void Setup_TimeScheme(int scheme_flg);
-void sol_update_RK(hos_cplx *u, double *t, double dt, char *dtflag);
+void sol_update_RK(hos_cplx *u, hos_real *t, hos_real dt, char *dtflag);
 
 /* hosm_2d_par.h:19-40 */
 void rhs_hos_setup(void);
 void rhs_hos_clean(void);
-void rhs_hos(hos_cplx *hrhs, hos_cplx *hu, double t);
-void Zvel(hos_cplx *hu, hos_cplx *hZvelM, hos_cplx *hZvelM2, hos_cplx *hZvel2M, hos_cplx *hZvel2M2, double t);
+void rhs_hos(hos_cplx *hrhs, hos_cplx *hu, hos_real t);
+void Zvel(hos_cplx *hu, hos_cplx *hZvelM, hos_cplx *hZvelM2, hos_cplx *hZvel2M, hos_cplx *hZvel2M2, hos_real t);
 void ZvelLinear(const hos_cplx *hu, hos_cplx *hZvelLinear);
-double Hamiltonian(const hos_cplx *heta, const hos_cplx *heta_t, const hos_cplx *hphi);
-double RampFun(const double t);
+hos_real Hamiltonian(const hos_cplx *heta, const hos_cplx *heta_t, const hos_cplx *hphi);
+hos_real RampFun(const hos_real t);
 
 /* operators_par.h:17-23 */
 void Dx(hos_cplx *hu, hos_cplx *hu_x);
@@ -79,8 +93,8 @@ void Sum(hos_cplx *hu1, hos_cplx *hu2, hos_cplx *hsum);
 void Dealias(hos_cplx *hu);
 
 /* fft_routines_2d_par.h:40-50 */
-void fft_2d(double *u, hos_cplx *hu, hos_plan plan);
-void ifft_2d(hos_cplx *hu, double *u, hos_plan plan);
+void fft_2d(hos_real *u, hos_cplx *hu, hos_plan plan);
+void ifft_2d(hos_cplx *hu, hos_real *u, hos_plan plan);
 
 /* additions of this boundary */
 void hos_ref_sync(void);               /* refresh the host copy of the state */

@@ -29,9 +29,11 @@ def test_library_exports_every_declared_symbol(hos):
     assert all(hasattr(lib, n) for n in srv)
 
 
-def test_reference_boundary_exports():
-    """libhos_ref.so exports the reference's own entry points (hos_ref.h)."""
-    lib = ctypes.CDLL(os.path.join(ROOT, "hos_amd", "libhos_ref.so"))
+@pytest.mark.parametrize("libname", ["libhos_ref.so", "libhos_ref_f32.so"])
+def test_reference_boundary_exports(libname):
+    """libhos_ref.so (double) and libhos_ref_f32.so (float: the reference built
+    without USE_DOUBLES) export the reference's own entry points (hos_ref.h)."""
+    lib = ctypes.CDLL(os.path.join(ROOT, "hos_amd", libname))
     names = ["Setup_TimeScheme", "sol_update_RK", "rhs_hos_setup", "rhs_hos_clean", "rhs_hos", "Zvel",
              "ZvelLinear", "Hamiltonian", "RampFun", "Dx", "Dy", "Dz", "Mult", "Sum", "Dealias", "fft_2d",
              "ifft_2d", "hos_local_size_2d_transposed", "hos_ref_sync", "hos_ref_host_modified",

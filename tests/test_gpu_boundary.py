@@ -297,3 +297,59 @@ def test_reference_input_pair_through_driver(tmp_path, oracle_mod):
     # the raw IC in data0 (main_2d_par.c:271-273)
     assert np.array_equal(read(os.path.join(d, "data0.1.h5"), "/eta", (n, n)), eta0)
     assert math.isfinite(float(rt[-1, 1]))
+
+
+def test_float_boundary_vs_oracle(oracle_mod):
+    """libhos_ref_f32.so: the same names over float / fftwf_complex (the
+    reference built without USE_DOUBLES, hosm_2d_par.h:31-38,
+    operators_par.h:24-30, fft_routines_2d_par.h:45-47,
+    time_schemes_par.h:22-24) on float host arrays, computing in the FP32
+    form with the GPU state in FP64 between sol_update_RK calls; against the
+    oracle in FP64 on the same (float-rounded) input.  Tolerance 2e-6
+    relative L2: the float rounding of inputs and outputs (6e-8) plus the
+    FP32 form (below 3e-7)."""
+    print("float boundary", flush=True)
+    nlevs = 2
+    L = ctypes.CDLL(os.path.join(BIN, "libhos_ref_f32.so"))
+    f, vp = ctypes.c_float, ctypes.c_void_p
+    L.rhs_hos.argtypes = [vp, vp, f]
+    L.rhs_hos.restype = None
+    L.sol_update_RK.argtypes = [vp, ctypes.POINTER(f), f, vp]
+    L.sol_update_RK.restype = None
+    L.Hamiltonian.argtypes = [vp, vp, vp]
+    L.Hamiltonian.restype = f
+    L.hos_local_size_2d_transposed.restype = ctypes.c_ssize_t
+    L.hos_local_size_2d_transposed.argtypes = [ctypes.c_ssize_t] * 2 + [vp] * 4
+    for name, ty, v in (("Nx", ctypes.c_int, Nx), ("Ny", ctypes.c_int, Ny), ("rampflg", ctypes.c_int, 0),
+                        ("Lx", f, Lx), ("Ly", f, Ly), ("g", f, 9.81), ("Tramp", f, 1.0),
+                        ("hos_nlevs", ctypes.c_int, nlevs), ("hos_runtime_log", ctypes.c_int, 0)):
+        ty.in_dll(L, name).value = v
+    assert ctypes.c_int.in_dll(L, "hos_precision").value == 1  # HOS_FP32 by default in the float build
+    L.hos_local_size_2d_transposed(Nx, Ny // 2 + 1, None, None, None, None)
+    L.Setup_TimeScheme(2)
+    L.rhs_hos_setup()
+    Lx32, Ly32, g32 = (float(np.float32(v)) for v in (Lx, Ly, 9.81))
+    o = oracle_mod.Oracle(Nx, Ny, nlevs, Lx32, Ly32, g=g32)
+    u64 = random_state(Nx, Ny, nlevs, amp=0.08, seed=35, Lx=Lx, Ly=Ly)
+    u = np.ascontiguousarray(u64.astype(np.complex64))  # fftwf_complex [2][Ny/2+1][Nx]
+    ur = np.ascontiguousarray(u.astype(np.complex128))
+    r = np.zeros_like(u)
+    L.rhs_hos(P(r), P(u), 0.0)
+    err = rel_l2(r, o.rhs_hos(ur))
+    print(f"  rhs_hos rel L2 {err:.2e}", flush=True)
+    assert err < 2e-6
+    d = np.zeros_like(u[0])
+    a = np.ascontiguousarray(u[1])
+    L.Dx(P(a), P(d))
+    assert rel_l2(d, o.Dx(np.ascontiguousarray(ur[1]))) < 2e-7
+    t = f(0.0)
+    tr = 0.0
+    for _ in range(4):
+        L.sol_update_RK(P(u), ctypes.byref(t), f(0.02), None)
+        tr, H = o.step(ur, tr, float(np.float32(0.02)))
+    L.hos_ref_sync()
+    err = rel_l2(u, ur)
+    print(f"  4 RK4 steps rel L2 {err:.2e}", flush=True)
+    assert err < 2e-6
+    assert abs(t.value - tr) < 1e-6
+    L.rhs_hos_clean()
