@@ -58,8 +58,9 @@ def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_s
     p = hos.params(Nx=w["Nx"], Ny=w["Ny"], nlevs=w["nlevs"], Lx=w["Lx"], Ly=w["Ly"], g=w["g"],
                    depth=w["depth"], precision=prec, compensated=1)
     s = hos.Solver(p, members)
+    # u0: one state for every member, or a list of one state per member
     for m in range(members):
-        s.set_state(u0, m)
+        s.set_state(u0[m] if isinstance(u0, list) else u0, m)
         s.set_member(m, w["depth"], dt)
     if tiles:
         # fixed launch shapes (c2r, r2c, pair y tiles; x-pass mode), no tuning trials
@@ -71,12 +72,15 @@ def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_s
             s.set_tile(tiles[4], hos.HOS_TILE_GROUPS)
         if len(tiles) > 5:
             s.set_tile(tiles[5], hos.HOS_TILE_PLAN)
+        if len(tiles) > 6:
+            s.set_tile(tiles[6], hos.HOS_TILE_FUSE)
         tiles = tuple(tiles[:3])
     else:
         tiles = s.autotune(3) if autotune else None
     xpass = s.get_tile(hos.HOS_TILE_XPASS)
     groups = s.get_tile(hos.HOS_TILE_GROUPS)
     plan = s.get_tile(hos.HOS_TILE_PLAN)
+    fuse = s.get_tile(hos.HOS_TILE_FUSE)
     s.use_graph(graph)
     s.step(warmup, dt)
     s.sync()
@@ -99,6 +103,7 @@ def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_s
         "xpass": "row per wave" if xpass == 2 else "rows per workgroup",
         "groups": groups,
         "plan": plan,
+        "fuse": fuse == 2,
     }
     if profile_steps:
         # same workload, stream-ordered launches with a HIP event pair around
@@ -196,10 +201,12 @@ def other_workloads(serve_seconds):
     return out
 
 
-# kernel name -> profiled kind (hos_kernel_stats): the RHS pass k_yr2c_rhs is
-# the "yr2c_pair" kind (it replaced the plain pair pass in the step)
-KIND_OF = [("k_yr2c_pair", "yr2c_pair"), ("k_yr2c_rhs", "yr2c_pair"), ("k_yr2c", "yr2c"), ("k_yc2r", "yc2r"), ("k_xfwdinv", "xfwdinv"),
-           ("k_xfwd_rhs", "xfwd_rhs"), ("k_xinv", "xinv"), ("k_copy_c64", "copy")]
+# kernel name -> profiled kind (hos_kernel_stats), first match: the RHS pass
+# k_yr2c_rhs is the "yr2c_pair" kind (it replaced the plain pair pass in the
+# step); k_yc2r_phi before its prefix k_yc2r
+KIND_OF = [("k_yr2c_pair", "yr2c_pair"), ("k_yr2c_rhs", "yr2c_pair"), ("k_yr2c", "yr2c"), ("k_yc2r_phi", "yc2r_phi"),
+           ("k_yc2r", "yc2r"), ("k_xfwdinv", "xfwdinv"), ("k_xfwd_rhs", "xfwd_rhs"), ("k_xinv", "xinv"),
+           ("k_copy_c64", "copy")]
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_1024_m5_x8.json")
 
 
@@ -251,7 +258,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="arrival window of the Poisson serving workload (0: skip it)")
     ap.add_argument("--tiles", type=str, default=None,
-                    help="fixed launch shapes c2r,r2c,pair[,xpass[,groups[,plan]]] for the batched form instead of the autotuner "
+                    help="fixed launch shapes c2r,r2c,pair[,xpass[,groups[,plan[,fuse]]]] for the batched form instead of the autotuner "
                          "(profiler runs: no tuning trials in the kernel statistics)")
     ap.add_argument("--calib", action="store_true", help="only the plain copy kernel (counter calibration)")
     ap.add_argument("--traffic", nargs=4, metavar=("FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
@@ -281,11 +288,14 @@ def main():
     Tp = hos.peak_period(w["g"], w["kp"], w["depth"])
     dt = Tp / 100.0
     u0 = make_state(hos, w)
+    # the batch: one sea state per member, seeds 1 .. members (member 0 is the
+    # seed-1 state the single-instance forms step)
+    batch = [u0] + [make_state(hos, dict(w, seed=w["seed"] + m)) for m in range(1, args.members)]
     print(f"[bench] device: {hos.device_name()}", file=sys.stderr, flush=True)
     graph = not args.no_graph
 
     fixed = [int(v) for v in args.tiles.split(",")] if args.tiles else None
-    f32 = run_form(hos, w, hos.HOS_FP32, args.members, args.steps, args.warmup, dt, u0, graph=graph,
+    f32 = run_form(hos, w, hos.HOS_FP32, args.members, args.steps, args.warmup, dt, batch, graph=graph,
                    profile_steps=args.profile_steps, tiles=fixed)
     print(f"[bench] fp32 x{args.members}: {f32['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     f1 = None
@@ -298,11 +308,14 @@ def main():
         print(f"[bench] fp64 x1: {f64['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     copy_gbps = hos.copy_bandwidth(256 << 20, 10)
 
-    # dominant kernel: largest summed duration in the profiled pass
+    # dominant kernel: the kernel (one template instance) with the largest
+    # summed duration in the profiled pass; the fused level passes are one
+    # instance per level 2 .. M-1
     ks = f32.get("kstats", {})
     roof = None
     if ks:
-        name = max(ks, key=lambda k: ks[k]["ms"])
+        inst = {n: (w["nlevs"] - 1 if n == "yc2r_phi" else 1) for n in ks}
+        name = max(ks, key=lambda k: ks[k]["ms"] / inst[k])
         k = ks[name]
         avg_ms = k["ms"] / max(k["launches"], 1)
         avg_bytes = k["bytes"] / max(k["launches"], 1)
@@ -349,11 +362,14 @@ def main():
         "data": "synthetic seeded JONSWAP sea states (hos_ic_jonswap), resident in HBM",
         "config": {
             "workload": f"FP32 batched form: {args.members} realisations of the 1024x1024 grid over 32x32 peak "
-                        "wavelengths, kp*h=1, M=5, JONSWAP gamma=5 kp*Hs/2=0.12 seed 1, RK4 dt=Tp/100, all "
-                        "members in the same launches; compared with the plain FP64 single instance",
+                        "wavelengths, kp*h=1, M=5, JONSWAP gamma=5 kp*Hs/2=0.12, "
+                        + (f"seeds 1..{args.members} (one sea state per member)" if args.members > 1 else "seed 1")
+                        + ", RK4 dt=Tp/100, all members in the same launches; compared with the plain FP64 single "
+                        "instance (seed 1)",
             "members": args.members, "graph": graph, "y_tiles_c2r_r2c_pair": f32["tiles"],
             "x_inverse_passes": f32["xpass"], "member_groups": f32["groups"],
             "rhs_fft_plan": {1: "16 8 8", 2: "8 8 4 4", 3: "4 4 4 4 4", 4: "16 16 4"}.get(f32["plan"], f32["plan"]),
+            "fused_level_passes": f32["fuse"],
             "form": "FP32 FFTs/products with Kahan order sums; FP64 state, RK combination, linear terms",
             "parallelism": "single GPU",
         },

@@ -34,6 +34,7 @@ struct hos_ctx {
     int tx[3] = {8, 4, 4};  // y tiles: c2r, r2c, pair
     int xwave = 0;          // x inverse passes one row per wave (where available)
     int plan = 0;           // FFT radix plan of the RHS pass (fft_stage_bits; plans 1-3 need Ny >= 512)
+    int fuse = 1;           // a W level's y c2r carries the next level's y r2c (k_yc2r_phi)
     // member groups: with G > 1, the active members split into G nearly equal
     // ranges that run as independent launch chains on their own streams (gst),
     // forked from and joined to the context stream each step
@@ -453,6 +454,9 @@ struct Eng {
     // pass (instantiated per order; its plans 1-3 in their own unit); defined
     // in hos_levels.hip
     int phi_level(int lev);
+    // y c2r of a level's outputs (fm, nf fields; field 0 d phi^(lev)/dz) with
+    // the y r2c of phi^(lev+1) (k_yc2r_phi), lev = 2..7
+    int yc2r_phi(const FieldMap &fm, int nf, int lev);
     int rhs_pair();
     int rhs_pair_plan();
 
@@ -467,8 +471,10 @@ struct Eng {
         const int n = level0(ops, fm, defer);
         int rc = xinv(Y, ops);
         if (!rc) rc = yc2r(fm, n, nzr());
+        bool have_phi = false;  // phi^(lev+1) already transformed (by yc2r_phi)
         for (int lev = 1; lev <= M - 1 && !rc; ++lev) {
-            rc = phi_level(lev);
+            if (!have_phi) rc = phi_level(lev);
+            have_phi = false;
             const int nd = M - lev;
             // the top derivative d^{M-lev} phi^(lev+1) is the W^(M) term k = M-lev-1
             if (!rc) rc = xfwdinv(0, 1, nd, defer ? dslot_w(M - lev - 1) : -1);
@@ -478,7 +484,14 @@ struct Eng {
                 f2.src[j - 1] = j;
                 f2.dst[j - 1] = ix.d[lev + 1][j];
             }
-            if (!rc && np > 0) rc = yc2r(f2, np);
+            if (rc || np == 0) continue;
+            // field 0 is d phi^(lev+1)/dz, which phi^(lev+2) reads
+            if (c->fuse && lev + 1 <= M - 1) {
+                rc = yc2r_phi(f2, np, lev + 1);
+                have_phi = true;
+            } else {
+                rc = yc2r(f2, np);
+            }
         }
         return rc;
     }
@@ -486,8 +499,9 @@ struct Eng {
     // RHS: the W levels (each field its own workgroups; the fields only the
     // RHS pass reads are left as x-intermediates), the RHS pass (their y c2r,
     // N_eta and N_phi, y r2c), and the x forward pass with the RK epilogue.
-    // (Fusing a level's y c2r with the next y r2c in one kernel serialises
-    // the fields of a level inside a workgroup and measured slower.)
+    // (Fusing all of a level's y c2r fields with the next y r2c in one
+    // workgroup serialises them and measured slower; k_yc2r_phi fuses the r2c
+    // behind the one field it reads.)
     int rhs(const cx<double> *Y, const RkArgs &rk)
     {
         int rc = w_levels(Y, true);

@@ -825,6 +825,13 @@ extern "C" int hos_set_tile(hos_ctx *c, int kind, int tx)
         drop_graphs(c);
         return HOS_OK;
     }
+    if (kind == HOS_TILE_FUSE) {
+        if (tx < 0 || tx > 2) return fail(HOS_E_INVALID, "fused level passes %d: 0 default, 1 off, 2 on", tx);
+        if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
+        c->fuse = tx != 1;
+        drop_graphs(c);
+        return HOS_OK;
+    }
     if (kind == HOS_TILE_GROUPS) {
         if (tx < 0 || tx > hos_ctx::MAXG) return fail(HOS_E_INVALID, "member groups %d: 0 default, 1 .. %d", tx, hos_ctx::MAXG);
         if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
@@ -860,6 +867,7 @@ extern "C" int hos_get_tile(hos_ctx *c, int kind)
     if (!c) return fail(HOS_E_INVALID, "null context");
     if (kind == HOS_TILE_XPASS) return c->xwave ? 2 : 1;
     if (kind == HOS_TILE_BAND) return c->band_on ? 2 : 1;
+    if (kind == HOS_TILE_FUSE) return c->fuse ? 2 : 1;
     if (kind == HOS_TILE_PLAN) return c->plan + 1;
     if (kind == HOS_TILE_GROUPS) return c->ngroups;
     if (kind < HOS_TILE_C2R || kind > HOS_TILE_PAIR) return fail(HOS_E_INVALID, "unknown tile kind %d", kind);

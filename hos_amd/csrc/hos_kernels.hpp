@@ -308,8 +308,9 @@ struct FieldMap {
 // the real and imaginary part of one complex inverse FFT, written to dst
 // [nx][NY].  buf holds NPR*NY complex values.  Every load of the tile is
 // issued before the first LDS write; one tile per workgroup.  Rows ky >= nzr
-// are zero (not read).
-template <class T, int LOGNY, int NPR>
+// are zero (not read).  KEEP: the transformed rows stay in buf (row pair p
+// in row p, x0+2p the real parts), published by the caller's barrier.
+template <class T, int LOGNY, int NPR, bool KEEP = false>
 __device__ __forceinline__ void c2r_tile(const cx<T> *__restrict__ src, T *__restrict__ dst, int x0, int nx,
                                          cx<T> *buf, cx<T> *twl, const cx<T> *__restrict__ twy, int nzr)
 {
@@ -368,6 +369,11 @@ __device__ __forceinline__ void c2r_tile(const cx<T> *__restrict__ src, T *__res
             for (int m = 0; m < E; ++m) {
                 d0[lane + 64 * m] = v[m].x;
                 d1[lane + 64 * m] = v[m].y;
+            }
+            if constexpr (KEEP) {
+                wave_sync();  // the FFT's last reads of the row precede these writes
+#pragma unroll
+                for (int m = 0; m < E; ++m) row[L::at(0, lane + 64 * m)] = v[m];
             }
         }
         return;
@@ -550,6 +556,16 @@ struct ProPhi {
         for (int k = 1; k <= LEV; ++k) f[k] = ph.vat(m, ix.d[LEV + 1 - k][k], x, y);
         return comp ? calc<true>(f) : calc<false>(f);
     }
+    // the same with the k = 1 field d phi^(LEV)/dz given rather than read
+    __device__ __forceinline__ F operator()(int m, int x, int y, F d1) const
+    {
+        F f[LEV + 1];
+        f[0] = ph.vat(m, ix.eta, x, y);
+        f[1] = d1;
+#pragma unroll
+        for (int k = 2; k <= LEV; ++k) f[k] = ph.vat(m, ix.d[LEV + 1 - k][k], x, y);
+        return comp ? calc<true>(f) : calc<false>(f);
+    }
 };
 
 // position of d^j phi^(a)/dz^j among the M(M+1)/2 fields a = 1..M, j = 1..M-a+1
@@ -634,6 +650,25 @@ struct ProZvel {
         }
     }
 };
+// the half spectra of the two real rows of each pair p < NPR from the
+// complex FFT in buf (Z(ky) and Z(NY-ky)), written transposed to out [Nk][nx]
+// (x0+2p, x0+2p+1 side by side); 16 consecutive ky of one pair per 16 lanes
+// (contiguous in LDS)
+template <class T, int LOGNY, int NPR>
+__device__ __forceinline__ void r2c_store(const cx<T> *buf, cx<T> *__restrict__ out, int x0, int nx)
+{
+    constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1;
+    constexpr int NO = ((NK + 15) / 16) * 16 * NPR;
+    for (int e = threadIdx.x; e < NO; e += NT) {
+        const int p = (e / 16) % NPR, ky = (e / (16 * NPR)) * 16 + e % 16;
+        if (ky >= NK) continue;
+        const cx<T> zk = buf[Lds<T, NY>::at(p, ky)];
+        const cx<T> zm = buf[Lds<T, NY>::at(p, (NY - ky) & (NY - 1))];
+        store2(out + (long)ky * nx + x0 + 2 * p, cx<T>{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)},
+               cx<T>{T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x)});
+    }
+}
+
 // r2c along y of x-rows x0 .. x0+2*NPR-1 of the field the prologue computes
 // on the fly: rows x0+2p and x0+2p+1 are the real and imaginary part of one
 // complex FFT; the half spectra are written transposed into out [Nk][nx].
@@ -644,7 +679,7 @@ __device__ __forceinline__ void r2c_tile(const PRO &pro, int m, cx<T> *__restric
 {
     using F = vec_t<T>;
     constexpr int V = Vec<T>::N;
-    constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1, NYV = NY / V, NTASK = NPR * NYV;
+    constexpr int NY = 1 << LOGNY, NYV = NY / V, NTASK = NPR * NYV;
     constexpr int PER = (NTASK + NT - 1) / NT;
     TwLds<T, NY, NT> tws;  // twiddles read first, staged in LDS with the fill
     tws.load(twy);
@@ -664,17 +699,7 @@ __device__ __forceinline__ void r2c_tile(const PRO &pro, int m, cx<T> *__restric
     }
     tws.store(twl);
     lds_fft<T, LOGNY, NPR, NT, false>(buf, twl);
-    // the two real rows of pair p at ky from Z(ky) and Z(NY-ky); 16 consecutive
-    // ky of one pair per 16 lanes (contiguous in LDS)
-    constexpr int NO = ((NK + 15) / 16) * 16 * NPR;
-    for (int e = threadIdx.x; e < NO; e += NT) {
-        const int p = (e / 16) % NPR, ky = (e / (16 * NPR)) * 16 + e % 16;
-        if (ky >= NK) continue;
-        const cx<T> zk = buf[Lds<T, NY>::at(p, ky)];
-        const cx<T> zm = buf[Lds<T, NY>::at(p, (NY - ky) & (NY - 1))];
-        store2(out + (long)ky * nx + x0 + 2 * p, cx<T>{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)},
-               cx<T>{T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x)});
-    }
+    r2c_store<T, LOGNY, NPR>(buf, out, x0, nx);
 }
 
 // r2c along y of x-rows x0 .. x0+NROW-1 of two fields the prologue computes
@@ -743,6 +768,59 @@ __global__ __launch_bounds__(NT) void k_yr2c_pair(PRO pro, cx<T> *__restrict__ I
     const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TXP;
     const int m = blockIdx.z;
     r2c_pair_tile<T, LOGNY, TXP, PRO>(pro, m, I + m * sI + dst * fsI, x0, nx, buf, twl, twy);
+}
+
+// y c2r pass of a W level's outputs with the next level's y r2c fused behind
+// field 0.  Field 0 is d phi^(LEV)/dz, the one output of the level that
+// phi^(LEV+1) (ProPhi<LEV>) reads: its workgroups keep the transformed tile in
+// LDS, form phi^(LEV+1) on the same x-rows from it, eta and the earlier
+// levels' fields, and do that field's y r2c into slot dst; the other fields'
+// workgroups do their c2r only.  Same arithmetic as k_yc2r then k_yr2c with
+// equal tiles, so the same bits; one launch and one read of field 0 fewer.
+// grid (Nx/TX, nfields, members)
+template <class T, int LOGNY, int TX, int LEV>
+__global__ __launch_bounds__(NT) void k_yc2r_phi(cx<T> *__restrict__ I, long sI, long fsI, T *__restrict__ P,
+                                                 long sP, long fsP, FieldMap fm, int nx,
+                                                 const cx<T> *__restrict__ twy, int nzr, ProPhi<T, LEV> pro, int dst)
+{
+    using F = vec_t<T>;
+    constexpr int V = Vec<T>::N;
+    constexpr int NY = 1 << LOGNY, NPR = TX / 2, NYV = NY / V, NTASK = NPR * NYV;
+    constexpr int PER = (NTASK + NT - 1) / NT;
+    using L = Lds<T, NY>;
+    __shared__ cx<T> buf[L::size(NPR)];
+    __shared__ cx<T> twl[NY / 2];
+    const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TX;
+    const int m = blockIdx.z;
+    const cx<T> *src = I + m * sI + fm.src[blockIdx.y] * fsI;
+    T *pd = P + m * sP + fm.dst[blockIdx.y] * fsP;
+    if (blockIdx.y != 0) {
+        c2r_tile<T, LOGNY, NPR>(src, pd, x0, nx, buf, twl, twy, nzr);
+        return;
+    }
+    c2r_tile<T, LOGNY, NPR, true>(src, pd, x0, nx, buf, twl, twy, nzr);
+    __syncthreads();  // the tile of d phi^(LEV)/dz in buf
+    // as r2c_tile's prologue; each task reads and rewrites its own elements
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = threadIdx.x + q * NT;
+        if ((NTASK % NT) == 0 || e < NTASK) {
+            const int p = e / NYV, y = (e % NYV) * V;
+            cx<T> *d = buf + L::at(p, y);
+            F da, db;
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                da[v] = d[v].x;
+                db[v] = d[v].y;
+            }
+            const F ra = pro(m, x0 + 2 * p, y, da);
+            const F rb = pro(m, x0 + 2 * p + 1, y, db);
+#pragma unroll
+            for (int v = 0; v < V; ++v) d[v] = {ra[v], rb[v]};
+        }
+    }
+    lds_fft<T, LOGNY, NPR, NT, false>(buf, twl);
+    r2c_store<T, LOGNY, NPR>(buf, I + m * sI + dst * fsI, x0, nx);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,6 +1,7 @@
 // hos_levels.hip -- the y r2c launches whose prologue is instantiated per
-// HOS level (phi^(lev+1), lev = 1..7) or per order (the RHS pass
-// k_yr2c_rhs, M = 1..8).
+// HOS level (phi^(lev+1), lev = 1..7: k_yr2c, and k_yc2r_phi, which fuses it
+// behind the previous level's y c2r) or per order (the RHS pass k_yr2c_rhs,
+// M = 1..8).
 // Compiled once per precision (HOS_T) and part (HOS_PART 0: levels, 1: the
 // RHS pass with the default FFT plan, 2: its other plans) by the Makefile, so
 // the several hundred kernel instantiations build in parallel translation
@@ -35,6 +36,43 @@ int Eng<T>::phi_level(int lev)
     return fail(HOS_E_INVALID, "level %d out of range", lev);
 }
 template int Eng<HOS_T>::phi_level(int);
+
+template <class T, int LEV>
+static int yc2r_phi_t(Eng<T> &e, const FieldMap &fm, int nf)
+{
+    hos_ctx *c = e.c;
+    ProPhi<T, LEV> pro{e.phys(), c->ix, e.comp()};
+    const int tx = c->tx[HOS_TILE_C2R];
+    LOGN_SWITCH(c->logny, {
+        TX_SWITCH(T, tx, {
+            dim3 grid(c->p.Nx / TXV, nf, e.nm);
+            // the c2r bytes of nf fields, and the r2c's: LEV inputs (field 0
+            // comes from LDS) and its output
+            Prof pf(c, e.S(), HOS_K_YC2R_PHI,
+                    (double)e.nm * sizeof(T) *
+                        (nf * (2.0 * c->Nk * c->p.Nx + c->p.Nx * (double)c->p.Ny) +
+                         LEV * (double)c->p.Nx * c->p.Ny + 2.0 * c->fs));
+            hipLaunchKernelGGL((k_yc2r_phi<T, LG, TXV, LEV>), grid, dim3(NT), 0, e.S(), e.I(), e.sI(), c->fsI, e.P(),
+                               e.sP(), c->fsP, fm, (int)c->ip, e.twy(), c->Nk, pro, 0);
+        })
+    });
+    return HOS_OK;
+}
+
+template <class T>
+int Eng<T>::yc2r_phi(const FieldMap &fm, int nf, int lev)
+{
+    switch (lev) {
+    case 2: return yc2r_phi_t<T, 2>(*this, fm, nf);
+    case 3: return yc2r_phi_t<T, 3>(*this, fm, nf);
+    case 4: return yc2r_phi_t<T, 4>(*this, fm, nf);
+    case 5: return yc2r_phi_t<T, 5>(*this, fm, nf);
+    case 6: return yc2r_phi_t<T, 6>(*this, fm, nf);
+    case 7: return yc2r_phi_t<T, 7>(*this, fm, nf);
+    }
+    return fail(HOS_E_INVALID, "fused level %d out of range", lev);
+}
+template int Eng<HOS_T>::yc2r_phi(const FieldMap &, int, int);
 #else
 template <class T, int MO, int PLAN>
 static int rhs_pair_t(Eng<T> &e)

@@ -210,7 +210,8 @@ int hos_step_times(hos_ctx *c, double *ms, int n);
 #define HOS_K_YR2C_PAIR 3   /* y r2c pass with the two-field RHS prologue         */
 #define HOS_K_XFWDINV 4     /* x forward + d^j/dz^j + x inverse (one W level)     */
 #define HOS_K_XFWD_RHS 5    /* x forward pass with the RHS / RK epilogue          */
-#define HOS_K_NKINDS 6
+#define HOS_K_YC2R_PHI 6    /* y c2r of a W level with the next level's y r2c     */
+#define HOS_K_NKINDS 7
 
 /* 1: record a HIP event pair around every stream-ordered kernel launch of
  * later hos_step calls (not graph replays) and accumulate, per kernel kind,
@@ -254,9 +255,15 @@ int hos_use_graph(hos_ctx *c, int enable);
  * (16 16 4); 2-4 need Ny >= 512.  Plans round differently; hos_autotune
  * tries them all. */
 #define HOS_TILE_PLAN 7
+/* HOS_TILE_FUSE: 1 off, 2 on (0: default, on).  On: the y c2r launch of a
+ * W level's outputs also does the next level's y r2c in the workgroups of
+ * the one output that r2c reads (d phi^(l)/dz, from LDS), using the c2r
+ * tile; same results, bit for bit, with one launch and one field read fewer
+ * per level from the second level on. */
+#define HOS_TILE_FUSE 8
 int hos_set_tile(hos_ctx *c, int kind, int tx);
 /* current setting of one kind (0-2: tx; HOS_TILE_XPASS: 1 or 2; HOS_TILE_GROUPS: 1..4;
- * HOS_TILE_BAND: 1 or 2; HOS_TILE_PLAN: 1..4), < 0 on error */
+ * HOS_TILE_BAND, HOS_TILE_FUSE: 1 or 2; HOS_TILE_PLAN: 1..4), < 0 on error */
 int hos_get_tile(hos_ctx *c, int kind);
 
 /* Autotuner: time nsteps RK steps of the active members for every valid

@@ -482,6 +482,42 @@ def test_band_of_masked_states_is_bit_identical(hos, prec, Nx, Ny, nlevs, nm, xm
     assert np.array_equal(res[1][0][0][cut], bad[0][cut])  # its masked modes kept their values
 
 
+@pytest.mark.parametrize("prec,Nx,Ny,nlevs,nm,c2r", [
+    (1, 1024, 512, 5, 4, 8),   # wave y passes, three fused levels, two member groups
+    (1, 256, 1024, 6, 2, 4),   # four fused levels, four x-rows per y tile
+    (1, 128, 256, 4, 3, 16),   # sixteen x-rows per y tile
+    (0, 256, 128, 5, 2, 8),    # FP64: the workgroup FFT in the y c2r
+])
+def test_fused_level_passes_are_bit_identical(hos, prec, Nx, Ny, nlevs, nm, c2r):
+    """A W level's y c2r launch that also does the next level's y r2c
+    (HOS_TILE_FUSE on) steps states and energies bit-identically to the
+    separate launches (off) with the same tiles, RHS and Zvel alike."""
+    print(f"fuse prec={prec} {Nx}x{Ny} nlevs={nlevs} members={nm} c2r={c2r}", flush=True)
+    Lx, Ly = 2 * np.pi * Nx / 32, 2 * np.pi * Ny / 32
+    p = hos.params(Nx=Nx, Ny=Ny, nlevs=nlevs, Lx=Lx, Ly=Ly, precision=prec, depth=1.3)
+    states = [random_state(Nx, Ny, nlevs, amp=0.05, slope=3.0, seed=90 + i, Lx=Lx, Ly=Ly) for i in range(nm)]
+    out = {}
+    for fuse in (1, 2):
+        s = hos.Solver(p, nm)
+        s.set_tile(fuse, hos.HOS_TILE_FUSE)
+        assert s.get_tile(hos.HOS_TILE_FUSE) == fuse
+        s.set_tile(c2r, hos.HOS_TILE_ALL)
+        s.use_graph(fuse == 2)
+        for i, u in enumerate(states):
+            s.set_state(u, i)
+        r = s.rhs(states[0])
+        z = s.zvel(states[1 % nm])
+        s.step(3, 0.02)
+        out[fuse] = (r, z, [(s.get_state(i), s.energy(i)) for i in range(nm)])
+        s.close()
+    assert np.array_equal(out[1][0], out[2][0])
+    for a, b in zip(out[1][1], out[2][1]):
+        assert np.array_equal(a, b)
+    for (u1, e1), (u2, e2) in zip(out[1][2], out[2][2]):
+        assert np.array_equal(u1, u2)
+        assert e1 == e2
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_rhs_fft_plans_vs_oracle(hos, oracle_mod, prec):
     """Every FFT radix plan of the RHS pass (HOS_TILE_PLAN 1-4: the built-in
