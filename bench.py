@@ -74,6 +74,8 @@ def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_s
             s.set_tile(tiles[5], hos.HOS_TILE_PLAN)
         if len(tiles) > 6:
             s.set_tile(tiles[6], hos.HOS_TILE_FUSE)
+        if len(tiles) > 7:
+            s.set_tile(tiles[7], hos.HOS_TILE_SKEW)
         tiles = tuple(tiles[:3])
     else:
         tiles = s.autotune(3) if autotune else None
@@ -81,6 +83,7 @@ def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_s
     groups = s.get_tile(hos.HOS_TILE_GROUPS)
     plan = s.get_tile(hos.HOS_TILE_PLAN)
     fuse = s.get_tile(hos.HOS_TILE_FUSE)
+    skew = s.get_tile(hos.HOS_TILE_SKEW)
     s.use_graph(graph)
     s.step(warmup, dt)
     s.sync()
@@ -104,6 +107,7 @@ def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_s
         "groups": groups,
         "plan": plan,
         "fuse": fuse == 2,
+        "skew": skew == 2 and groups > 1,
     }
     if profile_steps:
         # same workload, stream-ordered launches with a HIP event pair around
@@ -258,7 +262,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="arrival window of the Poisson serving workload (0: skip it)")
     ap.add_argument("--tiles", type=str, default=None,
-                    help="fixed launch shapes c2r,r2c,pair[,xpass[,groups[,plan[,fuse]]]] for the batched form instead of the autotuner "
+                    help="fixed launch shapes c2r,r2c,pair[,xpass[,groups[,plan[,fuse[,skew]]]]] for the batched form instead of the autotuner "
                          "(profiler runs: no tuning trials in the kernel statistics)")
     ap.add_argument("--calib", action="store_true", help="only the plain copy kernel (counter calibration)")
     ap.add_argument("--traffic", nargs=4, metavar=("FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
@@ -370,6 +374,7 @@ def main():
             "x_inverse_passes": f32["xpass"], "member_groups": f32["groups"],
             "rhs_fft_plan": {1: "16 8 8", 2: "8 8 4 4", 3: "4 4 4 4 4", 4: "16 16 4"}.get(f32["plan"], f32["plan"]),
             "fused_level_passes": f32["fuse"],
+            "offset_member_groups": f32["skew"],
             "form": "FP32 FFTs/products with Kahan order sums; FP64 state, RK combination, linear terms",
             "parallelism": "single GPU",
         },

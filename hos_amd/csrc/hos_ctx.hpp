@@ -42,6 +42,8 @@ struct hos_ctx {
     int ngroups = 1;
     hipStream_t gst[MAXG] = {};
     hipEvent_t gfork = nullptr, gjoin[MAXG] = {};
+    hipEvent_t gskew = nullptr;  // group 0's mark that group 1 starts behind
+    int skew = 1;                // two member groups run free within a call, offset (HOS_TILE_SKEW)
     bool f32 = false;
     long fs = 0;    // complex elements per spectral field  (Nk*Nx)
     long ip = 0;    // row pitch of the x-intermediates I    (Nx + 16)
@@ -225,7 +227,13 @@ struct Eng {
     hipStream_t st = nullptr;  // launch stream (null: the context's)
     int inflight = 0;        // members whose launches run concurrently with this one (0: nm)
     bool band = false;       // the stage input is a masked state: skip its masked modes
+    hipEvent_t mark = nullptr;  // recorded on S() at point mark_at of the RHS (levels 0..M-1, M: after them)
+    int mark_at = -1;
     hipStream_t S() const { return st ? st : c->stream; }
+    void mark_if(int at) const
+    {
+        if (mark && mark_at == at) (void)hipEventRecord(mark, S());
+    }
     // spectral rows of the stage input that may be nonzero, and the column
     // band of the loads (0: every column)
     int nzr() const { return band ? c->my : c->Nk; }
@@ -471,6 +479,7 @@ struct Eng {
         const int n = level0(ops, fm, defer);
         int rc = xinv(Y, ops);
         if (!rc) rc = yc2r(fm, n, nzr());
+        mark_if(0);
         bool have_phi = false;  // phi^(lev+1) already transformed (by yc2r_phi)
         for (int lev = 1; lev <= M - 1 && !rc; ++lev) {
             if (!have_phi) rc = phi_level(lev);
@@ -484,7 +493,10 @@ struct Eng {
                 f2.src[j - 1] = j;
                 f2.dst[j - 1] = ix.d[lev + 1][j];
             }
-            if (rc || np == 0) continue;
+            if (rc || np == 0) {
+                mark_if(lev);
+                continue;
+            }
             // field 0 is d phi^(lev+1)/dz, which phi^(lev+2) reads
             if (c->fuse && lev + 1 <= M - 1) {
                 rc = yc2r_phi(f2, np, lev + 1);
@@ -492,6 +504,7 @@ struct Eng {
             } else {
                 rc = yc2r(f2, np);
             }
+            mark_if(lev);
         }
         return rc;
     }
@@ -506,6 +519,7 @@ struct Eng {
     {
         int rc = w_levels(Y, true);
         if (rc) return rc;
+        mark_if(c->M);
         rc = rhs_pair();
         if (!rc) rc = xfwd_rhs(0, -1, Y, rk);
         return rc;

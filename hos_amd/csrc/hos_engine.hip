@@ -279,6 +279,7 @@ static void release(hos_ctx *c)
         if (c->gjoin[g]) (void)hipEventDestroy(c->gjoin[g]);
     }
     if (c->gfork) (void)hipEventDestroy(c->gfork);
+    if (c->gskew) (void)hipEventDestroy(c->gskew);
 }
 
 
@@ -450,7 +451,7 @@ static bool band_now(const hos_ctx *c)
 
 // one RK step of members m0 .. m0+n-1, launched on stream st
 template <class T>
-static int launch_step(hos_ctx *c, int m0, int n, hipStream_t st)
+static int launch_step(hos_ctx *c, int m0, int n, hipStream_t st, hipEvent_t mark, int mark_at)
 {
     Eng<T> e{c, n, m0, st};
     e.inflight = c->active;  // the other member groups step concurrently
@@ -472,6 +473,8 @@ static int launch_step(hos_ctx *c, int m0, int n, hipStream_t st)
         rk.hpart += (long)m0 * nparts;
         rk.vol += m0;
         rk.mean0 += m0;
+        e.mark = s == 0 ? mark : nullptr;
+        e.mark_at = mark_at;
         if (c->subdiag) {
             rk.mode = EPI_RK_SUBDIAG;
             rk.b_next = (s + 1 < c->S) ? c->b[s + 1][s] : 0.0;
@@ -503,11 +506,11 @@ static int launch_step(hos_ctx *c, int m0, int n, hipStream_t st)
     return rc;
 }
 
-static int launch_group(hos_ctx *c, int g, int ng, hipStream_t st)
+static int launch_group(hos_ctx *c, int g, int ng, hipStream_t st, hipEvent_t mark = nullptr, int mark_at = -1)
 {
     int m0 = 0, n = 0;
     group_range(c, g, ng, &m0, &n);
-    return c->f32 ? launch_step<float>(c, m0, n, st) : launch_step<double>(c, m0, n, st);
+    return c->f32 ? launch_step<float>(c, m0, n, st, mark, mark_at) : launch_step<double>(c, m0, n, st, mark, mark_at);
 }
 
 // one step, stream-ordered: one chain on the context stream, or two groups
@@ -615,7 +618,47 @@ static int step_enqueue(hos_ctx *c, int nsteps)
     }
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
     HIP_TRY(hipEventRecord(c->step_ev[0], c->stream));
-    for (int n = 0; n < nsteps && c->active > 0; ++n) {
+    const int ng = c->active > 0 ? groups_now(c) : 1;
+    const bool skew = ng == 2 && c->skew && nsteps > 0;
+    if (skew) {
+        // The two member groups run free over the nsteps (no join per step):
+        // their members are independent.  Group 1 starts when group 0's
+        // first RHS has done its W level M-2, so while group 0 is in its
+        // RHS pass (bound by instruction issue) group 1 is in its level
+        // passes (bound by memory), and the offset holds over the steps
+        // (equal halves, equal step times).  Step times are group 0's.
+        if (!c->gskew) HIP_TRY(hipEventCreateWithFlags(&c->gskew, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(c->gfork, c->stream));
+        for (int g = 0; g < ng; ++g) HIP_TRY(hipStreamWaitEvent(c->gst[g], c->gfork, 0));
+        for (int n = 0; n < nsteps; ++n) {
+            int rc = HOS_OK;
+            if (n == 0)  // stream-ordered: the mark is an event between two launches
+                rc = launch_group(c, 0, ng, c->gst[0], c->gskew, std::max(0, c->M - 2));
+            else if (c->use_graph)
+                HIP_TRY(hipGraphLaunch(c->gexec_by_active[graph_slot(c, ng, 0)], c->gst[0]));
+            else
+                rc = launch_group(c, 0, ng, c->gst[0]);
+            if (rc) return rc;
+            HIP_TRY(hipEventRecord(c->step_ev[n + 1], c->gst[0]));
+        }
+        for (int g = 1; g < ng; ++g) {
+            HIP_TRY(hipStreamWaitEvent(c->gst[g], c->gskew, 0));
+            for (int n = 0; n < nsteps; ++n) {
+                if (c->use_graph) {
+                    HIP_TRY(hipGraphLaunch(c->gexec_by_active[graph_slot(c, ng, g)], c->gst[g]));
+                } else {
+                    int rc = launch_group(c, g, ng, c->gst[g]);
+                    if (rc) return rc;
+                }
+            }
+        }
+        for (int g = 0; g < ng; ++g) {
+            HIP_TRY(hipEventRecord(c->gjoin[g], c->gst[g]));
+            HIP_TRY(hipStreamWaitEvent(c->stream, c->gjoin[g], 0));
+        }
+    }
+    const int serial = skew ? 0 : nsteps;  // steps still to enqueue
+    for (int n = 0; n < serial && c->active > 0; ++n) {
         if (c->use_graph) {
             int rc = replay_step(c);
             if (rc) return rc;
@@ -832,6 +875,12 @@ extern "C" int hos_set_tile(hos_ctx *c, int kind, int tx)
         drop_graphs(c);
         return HOS_OK;
     }
+    if (kind == HOS_TILE_SKEW) {
+        if (tx < 0 || tx > 2) return fail(HOS_E_INVALID, "offset member groups %d: 0 default, 1 off, 2 on", tx);
+        if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
+        c->skew = tx != 1;
+        return HOS_OK;
+    }
     if (kind == HOS_TILE_GROUPS) {
         if (tx < 0 || tx > hos_ctx::MAXG) return fail(HOS_E_INVALID, "member groups %d: 0 default, 1 .. %d", tx, hos_ctx::MAXG);
         if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
@@ -868,6 +917,7 @@ extern "C" int hos_get_tile(hos_ctx *c, int kind)
     if (kind == HOS_TILE_XPASS) return c->xwave ? 2 : 1;
     if (kind == HOS_TILE_BAND) return c->band_on ? 2 : 1;
     if (kind == HOS_TILE_FUSE) return c->fuse ? 2 : 1;
+    if (kind == HOS_TILE_SKEW) return c->skew ? 2 : 1;
     if (kind == HOS_TILE_PLAN) return c->plan + 1;
     if (kind == HOS_TILE_GROUPS) return c->ngroups;
     if (kind < HOS_TILE_C2R || kind > HOS_TILE_PAIR) return fail(HOS_E_INVALID, "unknown tile kind %d", kind);
@@ -981,22 +1031,27 @@ extern "C" int hos_autotune(hos_ctx *c, int nsteps, int *tiles_out)
         c->plan = best[5];
     }
     if (!rc && c->subdiag) {
-        // the other member-group counts (each group at least two members)
-        // with the best tiles, x-pass mode and FFT plan
+        // every member-group count (each group at least two members) with
+        // the best tiles, x-pass mode and FFT plan, in calls of at least 8
+        // steps: two groups run offset over a call (HOS_TILE_SKEW), which
+        // pays over several steps, not at the ends of a call
         for (int k = 0; k < 3; ++k) c->tx[k] = best[k];
         c->xwave = best[3];
         c->plan = best[5];
-        const int tried = best[4];
+        const int ns = nsteps;
+        nsteps = std::max(nsteps, 8);
+        double gbest = 1e30;
         for (int ng = 1; ng <= hos_ctx::MAXG && !rc; ++ng) {
-            if (ng == tried || (ng > 1 && c->active < 2 * ng)) continue;
+            if (ng > 1 && c->active < 2 * ng) continue;
             c->ngroups = ng;
             double ms = 0.0;
             rc = trial(ms);
-            if (!rc && ms < best_ms) {
-                best_ms = ms;
+            if (!rc && ms < gbest) {
+                gbest = ms;
                 best[4] = ng;
             }
         }
+        nsteps = ns;
     }
     c->use_graph = saved_graph;
     for (int k = 0; k < 3; ++k) c->tx[k] = best[k];

@@ -261,15 +261,25 @@ int hos_use_graph(hos_ctx *c, int enable);
  * tile; same results, bit for bit, with one launch and one field read fewer
  * per level from the second level on. */
 #define HOS_TILE_FUSE 8
+/* HOS_TILE_SKEW: 1 off, 2 on (0: default, on).  With two member groups
+ * (HOS_TILE_GROUPS), the groups' launch chains run free over the steps of
+ * one call (joined at its end, not after every step), and group 1 starts
+ * when group 0's first RHS has done its W level M-2: its memory-bound level
+ * passes then overlap group 0's issue-bound RHS pass.  Members are
+ * independent, so results are the same bits; hos_step_times reports group
+ * 0's steps.  Three or four groups join after every step either way.
+ * hos_autotune leaves it as set. */
+#define HOS_TILE_SKEW 9
 int hos_set_tile(hos_ctx *c, int kind, int tx);
 /* current setting of one kind (0-2: tx; HOS_TILE_XPASS: 1 or 2; HOS_TILE_GROUPS: 1..4;
- * HOS_TILE_BAND, HOS_TILE_FUSE: 1 or 2; HOS_TILE_PLAN: 1..4), < 0 on error */
+ * HOS_TILE_BAND, HOS_TILE_FUSE, HOS_TILE_SKEW: 1 or 2; HOS_TILE_PLAN: 1..4), < 0 on error */
 int hos_get_tile(hos_ctx *c, int kind);
 
 /* Autotuner: time nsteps RK steps of the active members for every valid
  * (c2r, r2c, pair) tile combination, then with the best tiles the other
  * x-pass mode, then every FFT radix plan of the RHS pass (Ny >= 512), then
- * the other member-group counts; keep the fastest and store it in an
+ * every member-group count (in calls of at least 8 steps, over which two
+ * offset groups, HOS_TILE_SKEW, show their gain); keep the fastest and store it in an
  * in-process cache keyed by (Nx, Ny, M, precision, active members, scheme
  * kind, band); later calls with the same key apply the cached choice at once.
  * The members' state, time, dt and energies are restored afterwards.

@@ -518,6 +518,40 @@ def test_fused_level_passes_are_bit_identical(hos, prec, Nx, Ny, nlevs, nm, c2r)
         assert e1 == e2
 
 
+@pytest.mark.parametrize("prec,ng,graph", [(1, 2, True), (1, 3, False), (0, 2, True)])
+def test_offset_member_groups_are_bit_identical(hos, prec, ng, graph):
+    """Member groups that run free over a call, the others offset behind
+    group 0 (HOS_TILE_SKEW on), step every member, states, energies and
+    times, as the groups joined after every step (off); the steps are timed
+    one by one either way, and calls of one step and of several agree."""
+    Nx, Ny, nlevs, nm = 128, 256, 4, 2 * ng + 1
+    print(f"skew prec={prec} groups={ng} graph={graph}", flush=True)
+    Lx, Ly = 2 * np.pi * 4, 2 * np.pi * 8
+    p = hos.params(Nx=Nx, Ny=Ny, nlevs=nlevs, Lx=Lx, Ly=Ly, precision=prec, depth=1.4)
+    states = [random_state(Nx, Ny, nlevs, amp=0.05, slope=3.0, seed=110 + i, Lx=Lx, Ly=Ly) for i in range(nm)]
+    dts = 0.02 + 0.001 * np.arange(nm)
+    out = {}
+    for skew in (1, 2):
+        s = hos.Solver(p, nm)
+        s.set_tile(ng, hos.HOS_TILE_GROUPS)
+        s.set_tile(skew, hos.HOS_TILE_SKEW)
+        assert s.get_tile(hos.HOS_TILE_SKEW) == skew
+        s.use_graph(graph)
+        for i, u in enumerate(states):
+            s.set_state(u, i)
+        s.step(5, dts)
+        st = s.step_times()
+        assert len(st) == 5 and all(t > 0 for t in st)
+        s.step(1, dts)
+        s.step(3, dts)
+        out[skew] = [(s.get_state(i), s.energy(i), s.get_time(i)) for i in range(nm)]
+        s.close()
+    for (u1, e1, t1), (u2, e2, t2) in zip(out[1], out[2]):
+        assert np.array_equal(u1, u2)
+        assert e1 == e2
+        assert t1 == t2
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_rhs_fft_plans_vs_oracle(hos, oracle_mod, prec):
     """Every FFT radix plan of the RHS pass (HOS_TILE_PLAN 1-4: the built-in
