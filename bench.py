@@ -76,6 +76,8 @@ def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_s
             s.set_tile(tiles[6], hos.HOS_TILE_FUSE)
         if len(tiles) > 7:
             s.set_tile(tiles[7], hos.HOS_TILE_SKEW)
+        if len(tiles) > 8:
+            s.set_tile(tiles[8], hos.HOS_TILE_DIAG)
         tiles = tuple(tiles[:3])
     else:
         tiles = s.autotune(3) if autotune else None
@@ -84,6 +86,7 @@ def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_s
     plan = s.get_tile(hos.HOS_TILE_PLAN)
     fuse = s.get_tile(hos.HOS_TILE_FUSE)
     skew = s.get_tile(hos.HOS_TILE_SKEW)
+    diag = s.get_tile(hos.HOS_TILE_DIAG)
     s.use_graph(graph)
     s.step(warmup, dt)
     s.sync()
@@ -106,8 +109,9 @@ def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_s
         "xpass": "row per wave" if xpass == 2 else "rows per workgroup",
         "groups": groups,
         "plan": plan,
-        "fuse": fuse == 2,
+        "fuse": fuse == 2 and diag != 2,
         "skew": skew == 2 and groups > 1,
+        "diag": diag == 2,
     }
     if profile_steps:
         # same workload, stream-ordered launches with a HIP event pair around
@@ -209,6 +213,7 @@ def other_workloads(serve_seconds):
 # k_yr2c_rhs is the "yr2c_pair" kind (it replaced the plain pair pass in the
 # step); k_yc2r_phi before its prefix k_yc2r
 KIND_OF = [("k_yr2c_pair", "yr2c_pair"), ("k_yr2c_rhs", "yr2c_pair"), ("k_yr2c", "yr2c"), ("k_yc2r_phi", "yc2r_phi"),
+           ("k_ydiag", "ydiag"),
            ("k_yc2r", "yc2r"), ("k_xfwdinv", "xfwdinv"), ("k_xfwd_rhs", "xfwd_rhs"), ("k_xinv", "xinv"),
            ("k_copy_c64", "copy")]
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_1024_m5_x8.json")
@@ -262,7 +267,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="arrival window of the Poisson serving workload (0: skip it)")
     ap.add_argument("--tiles", type=str, default=None,
-                    help="fixed launch shapes c2r,r2c,pair[,xpass[,groups[,plan[,fuse[,skew]]]]] for the batched form instead of the autotuner "
+                    help="fixed launch shapes c2r,r2c,pair[,xpass[,groups[,plan[,fuse[,skew[,diag]]]]]] for the batched form instead of the autotuner "
                          "(profiler runs: no tuning trials in the kernel statistics)")
     ap.add_argument("--calib", action="store_true", help="only the plain copy kernel (counter calibration)")
     ap.add_argument("--traffic", nargs=4, metavar=("FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
@@ -314,11 +319,11 @@ def main():
 
     # dominant kernel: the kernel (one template instance) with the largest
     # summed duration in the profiled pass; the fused level passes are one
-    # instance per level 2 .. M-1
+    # instance per level 2 .. M-1, the diagonal passes one per diagonal 1 .. M-1
     ks = f32.get("kstats", {})
     roof = None
     if ks:
-        inst = {n: (w["nlevs"] - 1 if n == "yc2r_phi" else 1) for n in ks}
+        inst = {n: (w["nlevs"] - 1 if n == "yc2r_phi" else (w["nlevs"] if n == "ydiag" else 1)) for n in ks}
         name = max(ks, key=lambda k: ks[k]["ms"] / inst[k])
         k = ks[name]
         avg_ms = k["ms"] / max(k["launches"], 1)
@@ -373,6 +378,7 @@ def main():
             "members": args.members, "graph": graph, "y_tiles_c2r_r2c_pair": f32["tiles"],
             "x_inverse_passes": f32["xpass"], "member_groups": f32["groups"],
             "rhs_fft_plan": {1: "16 8 8", 2: "8 8 4 4", 3: "4 4 4 4 4", 4: "16 16 4"}.get(f32["plan"], f32["plan"]),
+            "w_expansion": "diagonal passes" if f32["diag"] else "level passes",
             "fused_level_passes": f32["fuse"],
             "offset_member_groups": f32["skew"],
             "form": "FP32 FFTs/products with Kahan order sums; FP64 state, RK combination, linear terms",

@@ -35,6 +35,7 @@ struct hos_ctx {
     int xwave = 0;          // x inverse passes one row per wave (where available)
     int plan = 0;           // FFT radix plan of the RHS pass (fft_stage_bits; plans 1-3 need Ny >= 512)
     int fuse = 1;           // a W level's y c2r carries the next level's y r2c (k_yc2r_phi)
+    int diag = 0;           // W levels as diagonal passes (k_ydiag; HOS_TILE_DIAG)
     // member groups: with G > 1, the active members split into G nearly equal
     // ranges that run as independent launch chains on their own streams (gst),
     // forked from and joined to the context stream each step
@@ -420,8 +421,20 @@ struct Eng {
     }
 
     // x-intermediate slots of the fields the RHS pass transforms itself
-    // (deferred; see k_yr2c_rhs): after the working slots 0 .. max(2, M)-1
-    int dslot(int f) const { return std::max(2, c->M) + f; }
+    // (deferred; see k_yr2c_rhs): after the working slots 0 .. max(2, M)-1,
+    // or with the diagonal passes after slots 0 (phi^(L+1)), 1 (eta) and the
+    // M(M-1)/2 diagonal slots
+    int dslot(int f) const { return (c->diag ? diag_base(c->M) : std::max(2, c->M)) + f; }
+    static int diag_base(int M) { return 2 + M * (M - 1) / 2; }
+    // diagonal slot of d^j phi^(a)/dz^j, a + j <= M: slots 2.. in order of a,
+    // the M-a fields of each a consecutive (an x forward-inverse pass writes
+    // its outputs j = 1.. to consecutive slots)
+    int dg(int a, int j) const
+    {
+        int off = 2;
+        for (int b = 1; b < a; ++b) off += c->M - b;
+        return off + j - 1;
+    }
     // slot of the W^(M) term k (d^{k+1} phi^(M-k)/dz^{k+1}) among them
     int dslot_w(int k) const { return dslot(4 + k); }
 
@@ -467,6 +480,46 @@ struct Eng {
     int yc2r_phi(const FieldMap &fm, int nf, int lev);
     int rhs_pair();
     int rhs_pair_plan();
+    int rhs_pair_diag();
+    int rhs_pair_diag_plan();
+    // diagonal pass L (k_ydiag), L = 1..7; defined in hos_levels.hip
+    int ydiag(int L);
+
+    // the W expansion as diagonal passes (c->diag): level 0 as in w_levels;
+    // then for L = 1..M-1 the diagonal pass L (phi^(L+1) to slot 0, W^(L) to
+    // the physical grid, and in pass 1 the y c2r of eta) and the x forward-inverse pass of
+    // phi^(L+1) (d^j phi^(L+1)/dz^j to the diagonal slots, the top one, a
+    // W^(M) term, deferred).  The RHS pass then reads eta and W^(1) ..
+    // W^(M-1).  Each d^j phi^(a) is y transformed once, by the pass that
+    // consumes it, and never reaches the physical grid.
+    int diag_levels(const cx<double> *Y)
+    {
+        const HosIdx &ix = c->ix;
+        const int M = c->M;
+        XinvList ops;
+        ops.push_back({0, OP_ID, 0, 1});
+        ops.push_back({0, OP_DX, 0, dslot(0)});
+        ops.push_back({0, OP_DY, 0, dslot(1)});
+        ops.push_back({1, OP_DX, 0, dslot(2)});
+        ops.push_back({1, OP_DY, 0, dslot(3)});
+        for (int j = 1; j < M; ++j) ops.push_back({1, OP_DZ, j, dg(1, j)});
+        ops.push_back({1, OP_DZ, M, dslot_w(M - 1)});
+        int rc = xinv(Y, ops);
+        if (M == 1 && !rc) {  // no diagonal pass: eta's y c2r on its own
+            FieldMap fm{};
+            fm.src[0] = 1;
+            fm.dst[0] = ix.eta;
+            rc = yc2r(fm, 1, nzr());
+        }
+        mark_if(0);
+        for (int L = 1; L <= M - 1 && !rc; ++L) {
+            rc = ydiag(L);
+            // d^j phi^(L+1)/dz^j, j = 1..M-L: the W^(M) term j = M-L deferred
+            if (!rc) rc = xfwdinv(0, M - L > 1 ? dg(L + 1, 1) : 0, M - L, dslot_w(M - L - 1));
+            mark_if(L);
+        }
+        return rc;
+    }
 
     // physical fields of the W expansion from the stage input Y; defer: the
     // fields only the RHS pass reads stay x-intermediates (dslot)
@@ -517,7 +570,7 @@ struct Eng {
     // behind the one field it reads.)
     int rhs(const cx<double> *Y, const RkArgs &rk)
     {
-        int rc = w_levels(Y, true);
+        int rc = c->diag ? diag_levels(Y) : w_levels(Y, true);
         if (rc) return rc;
         mark_if(c->M);
         rc = rhs_pair();
@@ -527,7 +580,7 @@ struct Eng {
 
     // the RHS pass of order MO (k_yr2c_rhs) with FFT plan PLAN; TXP = half the
     // pair tile, at most what keeps two 16-byte tasks per thread
-    template <int MO, int PLAN>
+    template <int MO, int PLAN, bool WF = false>
     int yr2c_rhs()
     {
         RhsDeferred<MO> dfr;
@@ -540,20 +593,22 @@ struct Eng {
             drows += dfr.nzr[f];
         }
         dfr.nzo = band ? c->my : c->Nk;
-        const int nin = 1 + (MO - 1) * MO / 2;  // eta and d^j phi^(a), a + j <= M, from the physical grid
+        // eta and d^j phi^(a), a + j <= M (or W^(1) .. W^(M-1)), from the physical grid
+        const int nin = 1 + (WF ? MO - 1 : (MO - 1) * MO / 2);
         const int tx = c->tx[HOS_TILE_PAIR];
         LOGN_SWITCH(c->logny, {
             TX_SWITCH(T, tx, {
                 constexpr int V = Vec<T>::N;
                 constexpr int TXP0 = TXV / 2;
-                constexpr int TXP = (TXP0 > 2 && TXP0 * (1 << LG) / V > 2 * NT_RHS) ? 2 : TXP0;
+                constexpr int NTR = rhs_nt<T, WF>();
+                constexpr int TXP = (TXP0 > 2 && TXP0 * (1 << LG) / V > 2 * NTR) ? 2 : TXP0;
                 dim3 grid(c->p.Nx / TXP, 1, nm);
                 Prof pf(c, S(), HOS_K_YR2C_PAIR,
                         (double)nm * sizeof(T) * c->p.Nx * (nin * (double)c->p.Ny + 2.0 * drows + 4.0 * dfr.nzo));
                 if constexpr (PLAN != 0 && LG < PLAN_LOGN_MIN) {
                     return fail(HOS_E_INVALID, "FFT plan %d needs Ny >= %d", PLAN, 1 << PLAN_LOGN_MIN);
                 } else {
-                    hipLaunchKernelGGL((k_yr2c_rhs<T, LG, TXP, MO, PLAN>), grid, dim3(NT_RHS), 0, S(), phys(), c->ix,
+                    hipLaunchKernelGGL((k_yr2c_rhs<T, LG, TXP, MO, PLAN, WF>), grid, dim3(NTR), 0, S(), phys(), c->ix,
                                        (int)comp(), I(), sI(), c->fsI, dfr, 0, (int)c->ip, twy());
                 }
             })

@@ -454,6 +454,7 @@ struct HosIdx {
     int M;
     int eta, etax, etay, phix, phiy;
     int d[MAXM + 1][MAXM + 1];  // d[a][j] = d^j phi^(a) / dz^j, a=1..M, j=1..M-a+1
+    int w[MAXM + 1];            // W^(a), a = 1..M-1 (the diagonal passes' physical outputs)
     double invfact[MAXM + 1];   // 1/k!
 };
 
@@ -654,12 +655,12 @@ struct ProZvel {
 // complex FFT in buf (Z(ky) and Z(NY-ky)), written transposed to out [Nk][nx]
 // (x0+2p, x0+2p+1 side by side); 16 consecutive ky of one pair per 16 lanes
 // (contiguous in LDS)
-template <class T, int LOGNY, int NPR>
+template <class T, int LOGNY, int NPR, int NTT = NT>
 __device__ __forceinline__ void r2c_store(const cx<T> *buf, cx<T> *__restrict__ out, int x0, int nx)
 {
     constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1;
     constexpr int NO = ((NK + 15) / 16) * 16 * NPR;
-    for (int e = threadIdx.x; e < NO; e += NT) {
+    for (int e = threadIdx.x; e < NO; e += NTT) {
         const int p = (e / 16) % NPR, ky = (e / (16 * NPR)) * 16 + e % 16;
         if (ky >= NK) continue;
         const cx<T> zk = buf[Lds<T, NY>::at(p, ky)];
@@ -916,36 +917,47 @@ struct RhsPre {
     F m2, w2, w22;
 };
 
-// the physical fields d^j phi^(a), a + j <= M, at one task's points (d_pos order)
-template <class T, int M>
+// the physical fields the RHS pass reads besides eta, at one task's points:
+// d^j phi^(a), a + j <= M (d_pos order), or with WF (the diagonal passes ran)
+// W^(1) .. W^(M-1) themselves
+template <class T, int M, bool WF>
 struct RhsPhys {
-    static constexpr int NF = (M - 1) * M / 2;
+    static constexpr int NF = WF ? M - 1 : (M - 1) * M / 2;
     vec_t<T> d[NF > 0 ? NF : 1];
     __device__ __forceinline__ void load(const Phys<T> &ph, const HosIdx &ix, int m, int x, int y)
     {
+        if constexpr (WF) {
 #pragma unroll
-        for (int a = 1; a < M; ++a)
+            for (int a = 1; a < M; ++a) d[a - 1] = ph.vat(m, ix.w[a], x, y);
+        } else {
 #pragma unroll
-            for (int j = 1; a + j <= M; ++j) d[d_pos<M - 1>(a, j)] = ph.vat(m, ix.d[a][j], x, y);
+            for (int a = 1; a < M; ++a)
+#pragma unroll
+                for (int j = 1; a + j <= M; ++j) d[d_pos<M - 1>(a, j)] = ph.vat(m, ix.d[a][j], x, y);
+        }
     }
 };
 
-template <class T, int M, bool C>
-__device__ __forceinline__ RhsPre<vec_t<T>, C> rhs_pre(const HosIdx &ix, const RhsPhys<T, M> &ps, vec_t<T> e)
+template <class T, int M, bool C, bool WF>
+__device__ __forceinline__ RhsPre<vec_t<T>, C> rhs_pre(const HosIdx &ix, const RhsPhys<T, M, WF> &ps, vec_t<T> e)
 {
     using F = vec_t<T>;
     const F *d = ps.d;
     F W[M + 1];
 #pragma unroll
     for (int a = 1; a < M; ++a) {
-        F w = F(1);
-        KSum<F, C> sum;
+        if constexpr (WF) {
+            W[a] = d[a - 1];
+        } else {
+            F w = F(1);
+            KSum<F, C> sum;
 #pragma unroll
-        for (int k = 0; k < a; ++k) {
-            sum.add(w * T(ix.invfact[k]) * d[d_pos<M - 1>(a - k, k + 1)]);
-            w *= e;
+            for (int k = 0; k < a; ++k) {
+                sum.add(w * T(ix.invfact[k]) * d[d_pos<M - 1>(a - k, k + 1)]);
+                w *= e;
+            }
+            W[a] = sum.value();
         }
-        W[a] = sum.value();
     }
     RhsPre<F, C> r;
 #pragma unroll
@@ -1001,15 +1013,18 @@ __device__ __forceinline__ void rhs_phase_b(cx<T> *__restrict__ out, int x0, int
 
 
 // the physical fields are read before phase A (with the first deferred tiles,
-// one memory latency less per tile) where they are few: FP32 form, M <= 5;
-// their sums then wait in LDS (4 vectors per task, 5 compensated)
-template <class T, int M>
+// one memory latency less per tile) where they are few: FP32 form, at most
+// ten fields (M <= 5, or W^(a) fields at any M); their sums then wait in LDS
+// (4 vectors per task, 5 compensated)
+template <class T, int M, bool WF = false>
 constexpr bool rhs_early()
 {
-    return sizeof(T) == 4 && (M - 1) * M / 2 <= 10;
+    // over W^(a) fields the sums are a few products: read them after phase A
+    // and keep the LDS free (141 against 145 us per launch at 8 x 1024^2, M = 5)
+    return sizeof(T) == 4 && !WF && RhsPhys<T, M, WF>::NF <= 10;
 }
 
-template <class T, int LOGNY, int TXP, int M, bool C, int NTT, int PLAN>
+template <class T, int LOGNY, int TXP, int M, bool C, int NTT, int PLAN, bool WF>
 __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, const cx<T> *__restrict__ Im,
                                          long fsI, const RhsDeferred<M> &dfr, cx<T> *__restrict__ out, int m,
                                          int x0, int nx, cx<T> *buf, vec_t<T> *pbuf, cx<T> *twl,
@@ -1020,7 +1035,7 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
     constexpr int NY = 1 << LOGNY, NYV = NY / V, NPR = TXP / 2;
     constexpr int NTASK = TXP * NYV, PER = (NTASK + NTT - 1) / NTT;
     constexpr int ND = M + 4;
-    constexpr bool EARLY = rhs_early<T, M>();
+    constexpr bool EARLY = rhs_early<T, M, WF>();
     using L = Lds<T, NY>;
     // this thread's tasks: V consecutive y of x-row p (clamped past the end)
     int tp[PER], ty[PER];
@@ -1046,7 +1061,7 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
         // the physical fields first, the first batch of deferred tiles right
         // after them (loads complete in order: the sums wait for the physical
         // fields only, the tiles are still in flight)
-        RhsPhys<T, M> ps[PER];
+        RhsPhys<T, M, WF> ps[PER];
 #pragma unroll
         for (int q = 0; q < PER; ++q) ps[q].load(ph, ix, m, x0 + tp[q], ty[q]);
         if (NB > 1)
@@ -1057,7 +1072,7 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
         for (int q = 0; q < PER; ++q) {
             const int e = threadIdx.x + q * NTT;
             if ((NTASK % NTT) != 0 && e >= NTASK) continue;
-            const RhsPre<F, C> r = rhs_pre<T, M, C>(ix, ps[q], eta[q]);
+            const RhsPre<F, C> r = rhs_pre<T, M, C, WF>(ix, ps[q], eta[q]);
             F *o = pbuf + e;  // own slots only: no barrier
             o[0] = r.nl.s;
             o[NTASK] = r.m2;
@@ -1133,9 +1148,9 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
     } else {
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            RhsPhys<T, M> ps;
+            RhsPhys<T, M, WF> ps;
             ps.load(ph, ix, m, x0 + tp[q], ty[q]);
-            pre[q] = rhs_pre<T, M, C>(ix, ps, eta[q]);
+            pre[q] = rhs_pre<T, M, C, WF>(ix, ps, eta[q]);
         }
     }
     rhs_phase_b<T, LOGNY, TXP, M, C, NTT, PER, PLAN>(out, x0, nx, dfr.nzo, buf, twl, tp, ty, pre, ex, ey, px, py, wm);
@@ -1144,10 +1159,21 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
 // 512 threads: half the per-thread points (and accumulators) of a 256-thread
 // workgroup on the same tile
 constexpr int NT_RHS = 512;
+// The RHS pass over W^(a) fields (after the diagonal passes) holds far fewer
+// values per point, so in the FP32 form it runs in 256-thread workgroups (two
+// tasks per thread): more, smaller workgroups per CU hide its latency better
+// (8 x 1024^2, M = 5: 145 -> 133 us per launch); FP64 keeps 512 threads
+template <class T, bool WF>
+constexpr int rhs_nt()
+{
+    return WF && sizeof(T) == 4 ? 256 : NT_RHS;
+}
 
-// PLAN: the radix family of its row FFTs (fft_stage_bits; the autotuner's choice)
-template <class T, int LOGNY, int TXP, int M, int PLAN>
-__global__ __launch_bounds__(NT_RHS) void k_yr2c_rhs(Phys<T> ph, HosIdx ix, int comp, cx<T> *__restrict__ I, long sI,
+// PLAN: the radix family of its row FFTs (fft_stage_bits; the autotuner's choice).
+// WF: the physical inputs are W^(1) .. W^(M-1) (written by the diagonal
+// passes, k_ydiag) rather than the d^j phi^(a), a + j <= M
+template <class T, int LOGNY, int TXP, int M, int PLAN, bool WF = false>
+__global__ __launch_bounds__((rhs_nt<T, WF>())) void k_yr2c_rhs(Phys<T> ph, HosIdx ix, int comp, cx<T> *__restrict__ I, long sI,
                                                      long fsI, RhsDeferred<M> dfr, int dst, int nx,
                                                      const cx<T> *__restrict__ twy)
 {
@@ -1157,16 +1183,169 @@ __global__ __launch_bounds__(NT_RHS) void k_yr2c_rhs(Phys<T> ph, HosIdx ix, int 
     __shared__ cx<T> buf[Lds<T, NY>::size(TXP > 3 * (TXP / 2) ? TXP : 3 * (TXP / 2))];
     // the early sums (rhs_early): up to 5 vectors per task
     constexpr int NTASK = TXP * (NY / Vec<T>::N);
-    __shared__ vec_t<T> pbuf[rhs_early<T, M>() ? 5 * NTASK : 1];
+    __shared__ vec_t<T> pbuf[rhs_early<T, M, WF>() ? 5 * NTASK : 1];
     __shared__ cx<T> twl[NY / 2];
     const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TXP;
     const int m = blockIdx.z;
     const cx<T> *Im = I + m * sI;
     cx<T> *out = I + m * sI + dst * fsI;
     if (comp)
-        rhs_tile<T, LOGNY, TXP, M, true, NT_RHS, PLAN>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, pbuf, twl, twy);
+        rhs_tile<T, LOGNY, TXP, M, true, rhs_nt<T, WF>(), PLAN, WF>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, pbuf, twl,
+                                                                     twy);
     else
-        rhs_tile<T, LOGNY, TXP, M, false, NT_RHS, PLAN>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, pbuf, twl, twy);
+        rhs_tile<T, LOGNY, TXP, M, false, rhs_nt<T, WF>(), PLAN, WF>(ph, ix, Im, fsI, dfr, out, m, x0, nx, buf, pbuf,
+                                                                      twl, twy);
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal pass L of the W expansion (L = 1 .. M-1).  The fields with order +
+// derivative = L+1, F_k = d^k phi^(L+1-k)/dz^k (k = 1..L), are exactly the
+// ones both phi^(L+1) and W^(L) are made of:
+//   phi^(L+1) = - sum_{k=1}^{L} eta^k/k! F_k          (hosm_2d_par.c:302-317)
+//   W^(L)     =   sum_{k=1}^{L} eta^(k-1)/(k-1)! F_k  (hosm_2d_par.c:322-357)
+// so no F_k needs to reach the physical grid: this pass does their y c2r
+// itself straight from the x-intermediates (three at a time in LDS, the next
+// tiles loading during the current FFT, as the RHS pass does its deferred
+// fields), sums both at each point with terms in the reference's order (k
+// ascending), writes W^(L) to the physical grid for the RHS pass, and does
+// the y r2c of phi^(L+1) into slot dst for the next x forward-inverse pass.
+// Each F_k is then written once (as an x-intermediate) and read once, against
+// a y c2r write and two reads of the physical field (the level pass and the
+// RHS pass) otherwise.  grid (Nx/TXP, 1, members)
+// Pass 1 also does the y c2r of eta (field 0 of its batch), which every later
+// pass and the RHS pass read from the physical grid.
+template <int L>
+struct DiagArgs {
+    static constexpr int ETA = L == 1 ? 1 : 0;
+    static constexpr int NF = L + ETA;  // fields transformed: [eta,] F_1 .. F_L
+    int slot[NF];  // x-intermediate slots ([eta,] F_k)
+    int nzr[NF];   // rows that may be nonzero (a band of a masked state, or Nk)
+    int dst;       // slot of the y r2c of phi^(L+1)
+    int wf;        // physical field of W^(L)
+    int etaf;      // physical field of eta (pass 1)
+};
+
+// 256-thread workgroups: 53.7 against 56.6 us per pass at 8 x 1024^2, M = 5
+// with 512 (more independent workgroups per CU; their stages are barriers)
+constexpr int NT_DIAG = 256;
+
+template <class T, int LOGNY, int TXP, int LV, bool C, int NTT>
+__device__ __forceinline__ void diag_tile(const Phys<T> &ph, T *__restrict__ P, const HosIdx &ix,
+                                          const cx<T> *__restrict__ Im, long fsI, const DiagArgs<LV> &da,
+                                          cx<T> *__restrict__ out, int m, int x0, int nx, cx<T> *buf, cx<T> *twl,
+                                          const cx<T> *__restrict__ twy)
+{
+    using F = vec_t<T>;
+    constexpr int V = Vec<T>::N;
+    constexpr int NY = 1 << LOGNY, NYV = NY / V, NPR = TXP / 2;
+    constexpr int NTASK = TXP * NYV, PER = (NTASK + NTT - 1) / NTT;
+    using Lay = Lds<T, NY>;
+    // this thread's tasks: V consecutive y of x-row p (clamped past the end)
+    int tp[PER], ty[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = min((int)threadIdx.x + q * NTT, NTASK - 1);
+        tp[q] = e / NYV;
+        ty[q] = (e % NYV) * V;
+    }
+    // at most three fields per LDS batch, the batches as even as they come
+    constexpr int ETA = DiagArgs<LV>::ETA, NFT = DiagArgs<LV>::NF;
+    constexpr int NB = (NFT + 2) / 3, BF = (NFT + NB - 1) / NB, NFL = NFT - (NB - 1) * BF;
+    C2rBatch<T, LOGNY, NPR, NTT, BF> tb;   // batches 0 .. NB-2
+    C2rBatch<T, LOGNY, NPR, NTT, NFL> tl;  // batch NB-1
+    TwLds<T, NY, NTT> tws;
+    tws.load(twy);
+    if (NB > 1)
+        tb.load(Im, fsI, da.slot, da.nzr, x0, nx);
+    else
+        tl.load(Im, fsI, da.slot, da.nzr, x0, nx);
+    F eta[PER], ek[PER];
+    KSum<F, C> ps[PER], ws[PER];  // the sums of phi^(L+1) (negated last) and W^(L)
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        if constexpr (!ETA) eta[q] = ph.vat(m, ix.eta, x0 + tp[q], ty[q]);
+        ek[q] = F(1);  // eta^(k-1) before field k
+    }
+    tws.store(twl);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int nf = b + 1 < NB ? BF : NFL;
+        if (b + 1 < NB)
+            tb.fill(buf);
+        else
+            tl.fill(buf);
+        if (b + 2 < NB)
+            tb.load(Im, fsI, da.slot + (b + 1) * BF, da.nzr + (b + 1) * BF, x0, nx);
+        else if (b + 1 < NB)
+            tl.load(Im, fsI, da.slot + (b + 1) * BF, da.nzr + (b + 1) * BF, x0, nx);
+        if (nf == 3)
+            lds_fft<T, LOGNY, 3 * NPR, NTT, true>(buf, twl);
+        else if (nf == 2)
+            lds_fft<T, LOGNY, 2 * NPR, NTT, true>(buf, twl);
+        else
+            lds_fft<T, LOGNY, NPR, NTT, true>(buf, twl);
+#pragma unroll
+        for (int i = 0; i < BF; ++i) {
+            if (i >= nf) break;
+            const int k = b * BF + i + 1 - ETA;  // field F_k (k = 0: eta)
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                F val;
+                const cx<T> *z = buf + Lay::at(i * NPR + (tp[q] >> 1), ty[q]);
+#pragma unroll
+                for (int v = 0; v < V; ++v) val[v] = (tp[q] & 1) ? z[v].y : z[v].x;
+                if (ETA && k == 0) {
+                    eta[q] = val;
+                    continue;
+                }
+                ws[q].add(ek[q] * T(ix.invfact[k - 1]) * val);
+                ek[q] *= eta[q];
+                ps[q].add(ek[q] * T(ix.invfact[k]) * val);
+            }
+        }
+        __syncthreads();  // buf is refilled next
+    }
+    // W^(L) (and eta) to the physical grid; phi^(L+1) of x-rows x0+2p (real
+    // part) and x0+2p+1 (imaginary part) into row p of the r2c tile
+    T *wp = P + m * ph.sP + da.wf * ph.fsP;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = threadIdx.x + q * NTT;
+        if ((NTASK % NTT) != 0 && e >= NTASK) continue;
+        const long at = (long)(x0 + tp[q]) * ph.ny + ty[q];
+        *reinterpret_cast<F *>(wp + at) = ws[q].value();
+        if constexpr (ETA) *reinterpret_cast<F *>(P + m * ph.sP + da.etaf * ph.fsP + at) = eta[q];
+        const F phv = -ps[q].value();
+        cx<T> *o = buf + Lay::at(tp[q] >> 1, ty[q]);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            if (tp[q] & 1)
+                o[v].y = phv[v];
+            else
+                o[v].x = phv[v];
+        }
+    }
+    lds_fft<T, LOGNY, NPR, NTT, false>(buf, twl);
+    r2c_store<T, LOGNY, NPR, NTT>(buf, out, x0, nx);
+}
+
+template <class T, int LOGNY, int TXP, int LV>
+__global__ __launch_bounds__(NT_DIAG) void k_ydiag(Phys<T> ph, T *__restrict__ P, HosIdx ix, int comp,
+                                                  cx<T> *__restrict__ I, long sI, long fsI, DiagArgs<LV> da, int nx,
+                                                  const cx<T> *__restrict__ twy)
+{
+    static_assert(TXP % 2 == 0, "x-rows in pairs");
+    constexpr int NY = 1 << LOGNY, NPR = TXP / 2, NBUF = DiagArgs<LV>::NF < 3 ? DiagArgs<LV>::NF : 3;
+    __shared__ cx<T> buf[Lds<T, NY>::size(NBUF * NPR)];
+    __shared__ cx<T> twl[NY / 2];
+    const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TXP;
+    const int m = blockIdx.z;
+    const cx<T> *Im = I + m * sI;
+    cx<T> *out = I + m * sI + da.dst * fsI;
+    if (comp)
+        diag_tile<T, LOGNY, TXP, LV, true, NT_DIAG>(ph, P, ix, Im, fsI, da, out, m, x0, nx, buf, twl, twy);
+    else
+        diag_tile<T, LOGNY, TXP, LV, false, NT_DIAG>(ph, P, ix, Im, fsI, da, out, m, x0, nx, buf, twl, twy);
 }
 
 // ---------------------------------------------------------------------------

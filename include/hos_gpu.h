@@ -211,7 +211,8 @@ int hos_step_times(hos_ctx *c, double *ms, int n);
 #define HOS_K_XFWDINV 4     /* x forward + d^j/dz^j + x inverse (one W level)     */
 #define HOS_K_XFWD_RHS 5    /* x forward pass with the RHS / RK epilogue          */
 #define HOS_K_YC2R_PHI 6    /* y c2r of a W level with the next level's y r2c     */
-#define HOS_K_NKINDS 7
+#define HOS_K_YDIAG 7       /* diagonal pass: y c2r of a W diagonal, W^(L), y r2c of phi^(L+1) */
+#define HOS_K_NKINDS 8
 
 /* 1: record a HIP event pair around every stream-ordered kernel launch of
  * later hos_step calls (not graph replays) and accumulate, per kernel kind,
@@ -270,14 +271,25 @@ int hos_use_graph(hos_ctx *c, int enable);
  * 0's steps.  Three or four groups join after every step either way.
  * hos_autotune leaves it as set. */
 #define HOS_TILE_SKEW 9
+/* HOS_TILE_DIAG: 1 off, 2 on (0: default, on).  On: the W expansion runs as
+ * diagonal passes.  Pass L (L = 1..M-1) does the y c2r of the fields with
+ * order + derivative = L+1, d^k phi^(L+1-k)/dz^k (k = 1..L), straight from
+ * the x-intermediates, forms phi^(L+1) and W^(L) from them (the same fields
+ * make both, hosm_2d_par.c:302-357), writes W^(L) to the physical grid and
+ * does the y r2c of phi^(L+1); the RHS pass then reads W^(1)..W^(M-1)
+ * instead of the M(M-1)/2 derivative fields.  No derivative field reaches
+ * the physical grid.  Results agree with the mode off to rounding. */
+#define HOS_TILE_DIAG 10
 int hos_set_tile(hos_ctx *c, int kind, int tx);
 /* current setting of one kind (0-2: tx; HOS_TILE_XPASS: 1 or 2; HOS_TILE_GROUPS: 1..4;
- * HOS_TILE_BAND, HOS_TILE_FUSE, HOS_TILE_SKEW: 1 or 2; HOS_TILE_PLAN: 1..4), < 0 on error */
+ * HOS_TILE_BAND, HOS_TILE_FUSE, HOS_TILE_SKEW, HOS_TILE_DIAG: 1 or 2; HOS_TILE_PLAN: 1..4),
+ * < 0 on error */
 int hos_get_tile(hos_ctx *c, int kind);
 
 /* Autotuner: time nsteps RK steps of the active members for every valid
  * (c2r, r2c, pair) tile combination, then with the best tiles the other
- * x-pass mode, then every FFT radix plan of the RHS pass (Ny >= 512), then
+ * x-pass mode, then the other W-expansion form (HOS_TILE_DIAG), then every
+ * FFT radix plan of the RHS pass (Ny >= 512), then
  * every member-group count (in calls of at least 8 steps, over which two
  * offset groups, HOS_TILE_SKEW, show their gain); keep the fastest and store it in an
  * in-process cache keyed by (Nx, Ny, M, precision, active members, scheme

@@ -41,12 +41,16 @@ def test_fft_2d_vs_numpy(hos, Nx, Ny, prec):
 
 @pytest.mark.parametrize("nlevs", [0, 1, 2, 3, 4, 5, 7])
 @pytest.mark.parametrize("depth", [0.0, 0.7])
-def test_rhs_fp64_vs_oracle(hos, oracle_mod, nlevs, depth):
+@pytest.mark.parametrize("diag", [1, 2])
+def test_rhs_fp64_vs_oracle(hos, oracle_mod, nlevs, depth, diag):
+    """Both forms of the W expansion: level passes (HOS_TILE_DIAG 1) and
+    diagonal passes (2, the default)."""
     Nx, Ny = 64, 32
-    print(f"rhs fp64 nlevs={nlevs} depth={depth}", flush=True)
+    print(f"rhs fp64 nlevs={nlevs} depth={depth} diag={diag}", flush=True)
     Lx, Ly = 4 * np.pi, 2 * np.pi
     o = oracle_mod.Oracle(Nx, Ny, nlevs, Lx, Ly, depth=depth)
     s = make(hos, Nx, Ny, nlevs, 0, Lx=Lx, Ly=Ly, depth=depth)
+    s.set_tile(diag, hos.HOS_TILE_DIAG)
     u = random_state(Nx, Ny, nlevs, amp=0.08, seed=nlevs)
     ref = o.rhs_hos(u)
     got = s.rhs(u)
@@ -105,13 +109,16 @@ def test_xpass_modes_agree(hos, oracle_mod, Nx):
 
 
 @pytest.mark.parametrize("nlevs", [1, 2, 4, 5, 7])
-def test_rhs_fp32_vs_oracle(hos, oracle_mod, nlevs):
-    """FP32 RHS for M = 2..8: up to M = 5 the RHS pass reads its physical
-    fields before the deferred ones, above that after them."""
+@pytest.mark.parametrize("diag", [1, 2])
+def test_rhs_fp32_vs_oracle(hos, oracle_mod, nlevs, diag):
+    """FP32 RHS for M = 2..8, both W-expansion forms: with the level passes,
+    up to M = 5 the RHS pass reads its physical fields before the deferred
+    ones, above that after them."""
     Nx, Ny = 128, 64
-    print(f"rhs fp32 nlevs={nlevs}", flush=True)
+    print(f"rhs fp32 nlevs={nlevs} diag={diag}", flush=True)
     o = oracle_mod.Oracle(Nx, Ny, nlevs, 2 * np.pi, np.pi)
     s = make(hos, Nx, Ny, nlevs, 1, Ly=np.pi)
+    s.set_tile(diag, hos.HOS_TILE_DIAG)
     u = random_state(Nx, Ny, nlevs, amp=0.05, seed=7)
     err = rel_l2(s.rhs(u), o.rhs_hos(u))
     print(f"  rel L2 {err:.2e}", flush=True)
@@ -491,7 +498,8 @@ def test_band_of_masked_states_is_bit_identical(hos, prec, Nx, Ny, nlevs, nm, xm
 def test_fused_level_passes_are_bit_identical(hos, prec, Nx, Ny, nlevs, nm, c2r):
     """A W level's y c2r launch that also does the next level's y r2c
     (HOS_TILE_FUSE on) steps states and energies bit-identically to the
-    separate launches (off) with the same tiles, RHS and Zvel alike."""
+    separate launches (off) with the same tiles, RHS and Zvel alike (the level
+    passes: HOS_TILE_DIAG off)."""
     print(f"fuse prec={prec} {Nx}x{Ny} nlevs={nlevs} members={nm} c2r={c2r}", flush=True)
     Lx, Ly = 2 * np.pi * Nx / 32, 2 * np.pi * Ny / 32
     p = hos.params(Nx=Nx, Ny=Ny, nlevs=nlevs, Lx=Lx, Ly=Ly, precision=prec, depth=1.3)
@@ -499,6 +507,7 @@ def test_fused_level_passes_are_bit_identical(hos, prec, Nx, Ny, nlevs, nm, c2r)
     out = {}
     for fuse in (1, 2):
         s = hos.Solver(p, nm)
+        s.set_tile(1, hos.HOS_TILE_DIAG)
         s.set_tile(fuse, hos.HOS_TILE_FUSE)
         assert s.get_tile(hos.HOS_TILE_FUSE) == fuse
         s.set_tile(c2r, hos.HOS_TILE_ALL)
