@@ -1218,6 +1218,8 @@ template <int L>
 struct DiagArgs {
     static constexpr int ETA = L == 1 ? 1 : 0;
     static constexpr int NF = L + ETA;  // fields transformed: [eta,] F_1 .. F_L
+    // at most three fields per LDS batch, the batches as even as they come
+    static constexpr int NB = (NF + 2) / 3, BF = (NF + NB - 1) / NB, NFL = NF - (NB - 1) * BF;
     int slot[NF];  // x-intermediate slots ([eta,] F_k)
     int nzr[NF];   // rows that may be nonzero (a band of a masked state, or Nk)
     int dst;       // slot of the y r2c of phi^(L+1)
@@ -1248,9 +1250,8 @@ __device__ __forceinline__ void diag_tile(const Phys<T> &ph, T *__restrict__ P, 
         tp[q] = e / NYV;
         ty[q] = (e % NYV) * V;
     }
-    // at most three fields per LDS batch, the batches as even as they come
-    constexpr int ETA = DiagArgs<LV>::ETA, NFT = DiagArgs<LV>::NF;
-    constexpr int NB = (NFT + 2) / 3, BF = (NFT + NB - 1) / NB, NFL = NFT - (NB - 1) * BF;
+    using DA = DiagArgs<LV>;
+    constexpr int ETA = DA::ETA, NB = DA::NB, BF = DA::BF, NFL = DA::NFL;
     C2rBatch<T, LOGNY, NPR, NTT, BF> tb;   // batches 0 .. NB-2
     C2rBatch<T, LOGNY, NPR, NTT, NFL> tl;  // batch NB-1
     TwLds<T, NY, NTT> tws;
@@ -1335,7 +1336,7 @@ __global__ __launch_bounds__(NT_DIAG) void k_ydiag(Phys<T> ph, T *__restrict__ P
                                                   const cx<T> *__restrict__ twy)
 {
     static_assert(TXP % 2 == 0, "x-rows in pairs");
-    constexpr int NY = 1 << LOGNY, NPR = TXP / 2, NBUF = DiagArgs<LV>::NF < 3 ? DiagArgs<LV>::NF : 3;
+    constexpr int NY = 1 << LOGNY, NPR = TXP / 2, NBUF = DiagArgs<LV>::BF;  // rows of the largest batch
     __shared__ cx<T> buf[Lds<T, NY>::size(NBUF * NPR)];
     __shared__ cx<T> twl[NY / 2];
     const int x0 = xcd_tile(blockIdx.x, gridDim.x) * TXP;
