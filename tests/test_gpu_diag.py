@@ -96,12 +96,14 @@ def test_diag_rhs_fft_plans_vs_oracle(hos, oracle_mod, prec):
 
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("band", [1, 2])
-def test_diag_steps_match_mode_off(hos, prec, band):
+@pytest.mark.parametrize("nlevs", [0, 2, 4])
+def test_diag_steps_match_mode_off(hos, prec, band, nlevs):
     """Ten RK4 steps with the diagonal passes against the same steps with the
-    mode off, a masked state with the band of masked states on and off: FP64
-    to rounding, FP32 within its form's error; energies likewise."""
-    Nx, Ny, nlevs = 128, 128, 4
-    print(f"diag steps prec={prec} band={band}", flush=True)
+    mode off, a masked state with the band of masked states on and off, M = 1
+    (no diagonal pass: eta's own y c2r), 3 and 5: FP64 to rounding, FP32
+    within its form's error; energies likewise."""
+    Nx, Ny = 128, 128
+    print(f"diag steps prec={prec} band={band} nlevs={nlevs}", flush=True)
     u = random_state(Nx, Ny, nlevs, amp=0.06, seed=23)
     out = {}
     for diag in (1, 2):
