@@ -596,15 +596,22 @@ __device__ __forceinline__ void w_orders_v(const HosIdx &ix, F e, const F *d, F 
     }
 }
 
-// (W^2)^(K) = sum_{a,b>=1, a+b<=K} W^(a) W^(b)  (hosm_2d_par.c:397-442)
+// (W^2)^(K) = sum_{a,b>=1, a+b<=K} W^(a) W^(b)  (hosm_2d_par.c:397-442).
+// The reference sums both orders of every pair; W^(a) W^(b) + W^(b) W^(a) is
+// exactly 2 W^(a) W^(b) in floating point, so each pair a < b enters once,
+// doubled (6 products instead of 10 at K = 5), the orders a + b descending
+// (the smaller terms first)
 template <class F, int K, bool C>
 __device__ __forceinline__ F w2_upto_v(const F *W)
 {
     KSum<F, C> s;
 #pragma unroll
-    for (int a = 1; a < K; ++a)
+    for (int n = K; n >= 2; --n)
 #pragma unroll
-        for (int b = 1; a + b <= K; ++b) s.add(W[a] * W[b]);
+        for (int a = 1; 2 * a <= n; ++a) {
+            const int b = n - a;
+            s.add(a == b ? W[a] * W[a] : (W[a] + W[a]) * W[b]);
+        }
     return s.value();
 }
 
@@ -1045,8 +1052,10 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
         tp[q] = e / NYV;
         ty[q] = (e % NYV) * V;
     }
-    F eta[PER], ex[PER], ey[PER], px[PER], py[PER];
+    F eta[PER], ex[PER], ey[PER], px[PER], py[PER], wek[PER];
     KSum<F, C> wm[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) wek[q] = F(1);
     RhsPre<F, C> pre[PER];
     // deferred fields in batches of BF, the last one of NFL
     constexpr int BF = 3, NB = (ND + BF - 1) / BF, NFL = ND - (NB - 1) * BF;
@@ -1125,9 +1134,10 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
                 else if (f == 3) py[q] = val;
                 else {
                     // W^(M) term k = f-4: eta^k/k! d^{k+1} phi^(M-k)/dz^{k+1}, k ascending
-                    F ek = F(1);
-                    for (int k = 0; k < f - 4; ++k) ek *= eta[q];
-                    wm[q].add(ek * T(ix.invfact[f - 4]) * val);
+                    // (wek: eta^k, the running product, the same bits as a
+                    // product of k factors from 1)
+                    if (f > 4) wek[q] *= eta[q];
+                    wm[q].add(wek[q] * T(ix.invfact[f - 4]) * val);
                 }
             }
         }
