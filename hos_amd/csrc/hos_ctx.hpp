@@ -360,26 +360,31 @@ struct Eng {
         });
         return HOS_OK;
     }
-    // outputs j = 1..ndst to fields dst0+j-1, the last one to dtop if dtop >= 0
-    int xfwdinv(int src, int dst0, int ndst, int dtop = -1)
+    // outputs j = 1..ndst to fields dst0+j-1, the last one to dtop if dtop >= 0.
+    // spec (ndst = 1, dtop >= 0): the output d phi/dz stays in x-spectral
+    // space, rows ky < my only (what the mask keeps), for the RK epilogue
+    int xfwdinv(int src, int dst0, int ndst, int dtop = -1, bool spec = false)
     {
+        if (spec && (ndst != 1 || dtop < 0)) return fail(HOS_E_INVALID, "spectral x output: one output to dtop");
         const double scale = 1.0 / ((double)c->p.Nx * c->p.Ny);
+        const int nrows = spec ? c->my : c->Nk;
         LOGN_SWITCH(c->lognx, {
             constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
-            dim3 grid((c->Nk + ROWS - 1) / ROWS, 1, nm);
-            Prof pf(c, S(), HOS_K_XFWDINV, (double)nm * c->fs * (2.0 * sizeof(T) * (1 + ndst) + (sizeof(T) == 8 ? 8.0 : 0.0)));
+            dim3 grid((nrows + ROWS - 1) / ROWS, 1, nm);
+            Prof pf(c, S(), HOS_K_XFWDINV,
+                    (double)nm * nrows * c->p.Nx * (2.0 * sizeof(T) * (1 + ndst) + (sizeof(T) == 8 ? 8.0 : 0.0)));
             bool waved = false;
             if constexpr (x_wave<T>(LG)) {
                 if (c->xwave) {
-                    dim3 gw((c->Nk + XW - 1) / XW, 1, nm);
+                    dim3 gw((nrows + XW - 1) / XW, 1, nm);
                     hipLaunchKernelGGL((k_xfwdinv_w<T, LG>), gw, dim3(NT), 0, S(), I(), sI(), c->fsI, c->ip, src, dst0,
-                                       ndst, dtop, scale, kt(), twx());
+                                       ndst, dtop, scale, kt(), twx(), (int)spec, nrows);
                     waved = true;
                 }
             }
             if (!waved) {
                 hipLaunchKernelGGL((k_xfwdinv<T, LG, ROWS>), grid, dim3(NT), 0, S(), I(), sI(), c->fsI, c->ip, src, dst0,
-                                   ndst, dtop, scale, kt(), twx());
+                                   ndst, dtop, scale, kt(), twx(), (int)spec, nrows);
             }
         });
         return HOS_OK;
@@ -392,8 +397,9 @@ struct Eng {
         const int rows = c->lognx >= 10 ? 1 : (1 << (10 - c->lognx));
         return (rhs_rows(rk) + rows - 1) / rows;
     }
-    // ib < 0: the interleaved pair of yr2c_pair at field ia
-    int xfwd_rhs(int ia, int ib, const cx<double> *Y, const RkArgs &rk)
+    // ib < 0: the interleaved pair of yr2c_pair at field ia.  it >= 0: slot
+    // of the x spectrum of d phi^(M)/dz (diagonal passes), added to N_eta
+    int xfwd_rhs(int ia, int ib, const cx<double> *Y, const RkArgs &rk, int it = -1)
     {
         const double scale = 1.0 / ((double)c->p.Nx * c->p.Ny);
         const int nz = rhs_rows(rk);
@@ -413,9 +419,11 @@ struct Eng {
         LOGN_SWITCH(c->lognx, {
             constexpr int ROWS = LG >= 10 ? 1 : (1 << (10 - LG));
             dim3 grid((nz + ROWS - 1) / ROWS, 1, nm);
-            Prof pf(c, S(), HOS_K_XFWD_RHS, (double)nm * (2.0 * 2.0 * sizeof(T) * nz * c->p.Nx + 16.0 * f64 * kept));
+            const double tbytes = it >= 0 ? 2.0 * sizeof(T) * std::min(nz, c->my) * c->p.Nx : 0.0;
+            Prof pf(c, S(), HOS_K_XFWD_RHS,
+                    (double)nm * (2.0 * 2.0 * sizeof(T) * nz * c->p.Nx + tbytes + 16.0 * f64 * kept));
             hipLaunchKernelGGL((k_xfwd_rhs<T, LG, ROWS>), grid, dim3(NT), 0, S(), I(), sI(), c->fsI, c->ip, ia,
-                               ib < 0 ? ia : ib, ib < 0 ? 1 : 0, Y, sY(), scale, kt(), rk, twx(), nz);
+                               ib < 0 ? ia : ib, ib < 0 ? 1 : 0, Y, sY(), scale, kt(), rk, twx(), nz, it);
         });
         return HOS_OK;
     }
@@ -503,7 +511,9 @@ struct Eng {
         ops.push_back({1, OP_DX, 0, dslot(2)});
         ops.push_back({1, OP_DY, 0, dslot(3)});
         for (int j = 1; j < M; ++j) ops.push_back({1, OP_DZ, j, dg(1, j)});
-        ops.push_back({1, OP_DZ, M, dslot_w(M - 1)});
+        // the W^(M) terms k = 1..M-1 for the RHS pass (k = M-1: d^M phi^(1)/dz^M);
+        // at M = 1 the only term, d phi/dz, cancels against W^(1) in N_eta
+        if (M >= 2) ops.push_back({1, OP_DZ, M, dslot_w(M - 1)});
         int rc = xinv(Y, ops);
         if (M == 1 && !rc) {  // no diagonal pass: eta's y c2r on its own
             FieldMap fm{};
@@ -514,8 +524,11 @@ struct Eng {
         mark_if(0);
         for (int L = 1; L <= M - 1 && !rc; ++L) {
             rc = ydiag(L);
-            // d^j phi^(L+1)/dz^j, j = 1..M-L: the W^(M) term j = M-L deferred
-            if (!rc) rc = xfwdinv(0, M - L > 1 ? dg(L + 1, 1) : 0, M - L, dslot_w(M - L - 1));
+            // d^j phi^(L+1)/dz^j, j = 1..M-L: the W^(M) term j = M-L deferred.
+            // At L = M-1 that term is d phi^(M)/dz, which enters N_eta alone
+            // (no eta factor, no product): it stays in x-spectral space and the
+            // RK epilogue adds it (xfwd_rhs)
+            if (!rc) rc = xfwdinv(0, M - L > 1 ? dg(L + 1, 1) : 0, M - L, dslot_w(M - L - 1), L == M - 1);
             mark_if(L);
         }
         return rc;
@@ -574,7 +587,7 @@ struct Eng {
         if (rc) return rc;
         mark_if(c->M);
         rc = rhs_pair();
-        if (!rc) rc = xfwd_rhs(0, -1, Y, rk);
+        if (!rc) rc = xfwd_rhs(0, -1, Y, rk, c->diag && c->M >= 2 ? dslot_w(0) : -1);
         return rc;
     }
 
@@ -585,11 +598,14 @@ struct Eng {
     {
         RhsDeferred<MO> dfr;
         // the gradients (0..3) and d^M phi^(1)/dz^M (term M-1) come from the
-        // level-0 x inverse pass: a band of a masked state
+        // level-0 x inverse pass: a band of a masked state.  Over W fields
+        // (WF) the term k = 0 is added in spectral space by the epilogue:
+        // fields 4.. are the terms k = 1..M-1
+        constexpr int K0 = WF ? 1 : 0;
         double drows = 0.0;
-        for (int f = 0; f < MO + 4; ++f) {
-            dfr.slot[f] = dslot(f);
-            dfr.nzr[f] = (f < 4 || f == MO + 3) ? nzr() : c->Nk;
+        for (int f = 0; f < MO + 4 - K0; ++f) {
+            dfr.slot[f] = f < 4 ? dslot(f) : dslot(f + K0);
+            dfr.nzr[f] = (f < 4 || f == MO + 3 - K0) ? nzr() : c->Nk;
             drows += dfr.nzr[f];
         }
         dfr.nzo = band ? c->my : c->Nk;

@@ -663,13 +663,14 @@ struct ProZvel {
 // (x0+2p, x0+2p+1 side by side); 16 consecutive ky of one pair per 16 lanes
 // (contiguous in LDS)
 template <class T, int LOGNY, int NPR, int NTT = NT>
-__device__ __forceinline__ void r2c_store(const cx<T> *buf, cx<T> *__restrict__ out, int x0, int nx)
+__device__ __forceinline__ void r2c_store(const cx<T> *buf, cx<T> *__restrict__ out, int x0, int nx, int nzo = -1)
 {
     constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1;
     constexpr int NO = ((NK + 15) / 16) * 16 * NPR;
+    if (nzo < 0 || nzo > NK) nzo = NK;  // default: every row
     for (int e = threadIdx.x; e < NO; e += NTT) {
         const int p = (e / 16) % NPR, ky = (e / (16 * NPR)) * 16 + e % 16;
-        if (ky >= NK) continue;
+        if (ky >= nzo) continue;  // nzo <= NK
         const cx<T> zk = buf[Lds<T, NY>::at(p, ky)];
         const cx<T> zm = buf[Lds<T, NY>::at(p, (NY - ky) & (NY - 1))];
         store2(out + (long)ky * nx + x0 + 2 * p, cx<T>{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)},
@@ -1041,7 +1042,10 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
     constexpr int V = Vec<T>::N;
     constexpr int NY = 1 << LOGNY, NYV = NY / V, NPR = TXP / 2;
     constexpr int NTASK = TXP * NYV, PER = (NTASK + NTT - 1) / NTT;
-    constexpr int ND = M + 4;
+    // deferred fields: the gradients and the W^(M) terms k = K0..M-1 (over W
+    // fields the term k = 0, d phi^(M)/dz, is added in spectral space by the
+    // RK epilogue)
+    constexpr int K0 = WF ? 1 : 0, ND = M + 4 - K0;
     constexpr bool EARLY = rhs_early<T, M, WF>();
     using L = Lds<T, NY>;
     // this thread's tasks: V consecutive y of x-row p (clamped past the end)
@@ -1133,11 +1137,11 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
                 else if (f == 2) px[q] = val;
                 else if (f == 3) py[q] = val;
                 else {
-                    // W^(M) term k = f-4: eta^k/k! d^{k+1} phi^(M-k)/dz^{k+1}, k ascending
-                    // (wek: eta^k, the running product, the same bits as a
-                    // product of k factors from 1)
-                    if (f > 4) wek[q] *= eta[q];
-                    wm[q].add(wek[q] * T(ix.invfact[f - 4]) * val);
+                    // W^(M) term k = f-4+K0: eta^k/k! d^{k+1} phi^(M-k)/dz^{k+1}, k
+                    // ascending (wek: eta^k, the running product, the same bits
+                    // as a product of k factors from 1)
+                    if (f > 4 || K0) wek[q] *= eta[q];
+                    wm[q].add(wek[q] * T(ix.invfact[f - 4 + K0]) * val);
                 }
             }
         }
@@ -1233,6 +1237,7 @@ struct DiagArgs {
     int slot[NF];  // x-intermediate slots ([eta,] F_k)
     int nzr[NF];   // rows that may be nonzero (a band of a masked state, or Nk)
     int dst;       // slot of the y r2c of phi^(L+1)
+    int nzo;       // rows of that r2c written (the last pass: the rows the mask keeps)
     int wf;        // physical field of W^(L)
     int etaf;      // physical field of eta (pass 1)
 };
@@ -1337,7 +1342,7 @@ __device__ __forceinline__ void diag_tile(const Phys<T> &ph, T *__restrict__ P, 
         }
     }
     lds_fft<T, LOGNY, NPR, NTT, false>(buf, twl);
-    r2c_store<T, LOGNY, NPR, NTT>(buf, out, x0, nx);
+    r2c_store<T, LOGNY, NPR, NTT>(buf, out, x0, nx, da.nzo);
 }
 
 template <class T, int LOGNY, int TXP, int LV>
@@ -1368,7 +1373,7 @@ __global__ __launch_bounds__(NT_DIAG) void k_ydiag(Phys<T> ph, T *__restrict__ P
 template <class T, int LOGNX, int ROWS>
 __global__ __launch_bounds__(NT) void k_xfwdinv(cx<T> *__restrict__ I, long sI, long fs, long ip, int src, int dst0,
                                                 int ndst, int dtop, double scale, KTab kt,
-                                                const cx<T> *__restrict__ twx)
+                                                const cx<T> *__restrict__ twx, int spec, int nrows)
 {
     using G = XGeom<T, LOGNX>;
     static_assert(G::ROWS == ROWS, "x rows");
@@ -1404,6 +1409,19 @@ __global__ __launch_bounds__(NT) void k_xfwdinv(cx<T> *__restrict__ I, long sI, 
     lds_fft<T, LOGNX, ROWS, NT, false>(W, twx);
 #pragma unroll
     for (int q = 0; q < PER; ++q) s[q] = W[L::at((threadIdx.x + q * NT) / NX, (threadIdx.x + q * NT) % NX)];
+    if (spec) {
+        // the one output, d phi/dz (ndst = 1), left in x-spectral space for
+        // the RK epilogue: rows ky < nrows only
+        cx<T> *out = base + dtop * fs;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = threadIdx.x + q * NT;
+            const int kx = e % NX, ky = ky0 + e / NX;
+            const T mul = T(R(scale) * kz[q] * th[q]);
+            if (ky < nrows) out[(long)ky * ip + kx] = {mul * s[q].x, mul * s[q].y};
+        }
+        return;
+    }
     R kj[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) kj[q] = R(scale);
@@ -1470,10 +1488,13 @@ struct RkArgs {
 // nonlinear terms na (eta_t) and nb (phi_t) and the RK stage update
 // (time_schemes_par.c:38-67 / sol_update_RK), FP64.  hsum accumulates the
 // stage-0 Hamiltonian terms of the point.
+// ta: the x spectrum of d phi^(M)/dz, scaled (the diagonal passes leave this
+// term of W^(M), the one that enters N_eta linearly with no eta factor, in
+// spectral space; zero otherwise)
 template <class T>
 __device__ __forceinline__ void rk_epilogue(const RkArgs &rk, const KTab &kt, int m, long idx, int kx, int ky, int NX,
                                             cx<T> na, cx<T> nb, double scale, double R, double dt,
-                                            const cx<double> *y, long fsd, double &hsum)
+                                            const cx<double> *y, long fsd, double &hsum, cx<T> ta)
 {
     if (rk.mode == EPI_STORE_SPEC2) {
         cx<double> *k = rk.K + m * rk.sK;
@@ -1492,7 +1513,8 @@ __device__ __forceinline__ void rk_epilogue(const RkArgs &rk, const KTab &kt, in
     const double kz = sqrt(kxv * kxv + kyv * kyv);
     const double w1 = kz * (h > 0.0 ? tanh(kz * h) : 1.0);
     // eta_t = W^(1) + R*(rest), phi_t = -g eta + R*(rest)  (hosm_2d_par.c:211-229)
-    cx<double> ke = {w1 * yp.x + R * scale * double(na.x), w1 * yp.y + R * scale * double(na.y)};
+    cx<double> ke = {w1 * yp.x + R * (scale * double(na.x) + double(ta.x)),
+                     w1 * yp.y + R * (scale * double(na.y) + double(ta.y))};
     cx<double> kp = {-rk.g * ye.x + R * scale * double(nb.x), -rk.g * ye.y + R * scale * double(nb.y)};
     if (masked(kx, ky, NX, kt.mx, kt.my)) {
         ke = {0.0, 0.0};
@@ -1554,7 +1576,7 @@ __device__ __forceinline__ void rk_epilogue(const RkArgs &rk, const KTab &kt, in
 template <class T, int LOGNX, int ROWS>
 __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, long sI, long fs, long ip, int iA,
                                                  int iB, int ilv, const cx<double> *Y, long sY, double scale,
-                                                 KTab kt, RkArgs rk, const cx<T> *__restrict__ twx, int nzr)
+                                                 KTab kt, RkArgs rk, const cx<T> *__restrict__ twx, int nzr, int iT)
 {
     using G = XGeom<T, LOGNX>;
     static_assert(G::ROWS == ROWS, "x rows");
@@ -1608,7 +1630,11 @@ __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, lo
         const int r = e / NX, kx = e % NX, ky = ky0 + r;
         if (ky >= nzr) continue;
         const long idx = (long)ky * NX + kx;
-        rk_epilogue(rk, kt, m, idx, kx, ky, NX, S[L::at(r, kx)], S[L::at(ROWS + r, kx)], scale, R, dt, y, fsd, hsum);
+        // the spectral term (iT >= 0) exists on the rows the mask keeps
+        cx<T> ta = {T(0), T(0)};
+        if (iT >= 0 && ky < kt.my) ta = I[m * sI + iT * fs + (long)ky * ip + kx];
+        rk_epilogue(rk, kt, m, idx, kx, ky, NX, S[L::at(r, kx)], S[L::at(ROWS + r, kx)], scale, R, dt, y, fsd, hsum,
+                    ta);
     }
     if (rk.hpart && rk.stage == 0 && rk.mode != EPI_STORE_SPEC2) {
         // deterministic block reduction: wave shuffle, then fixed-order sum of waves
@@ -1708,7 +1734,7 @@ __global__ __launch_bounds__(NT) void k_xinv_w(const cx<double> *__restrict__ Y,
 template <class T, int LOGNX>
 __global__ __launch_bounds__(NT) void k_xfwdinv_w(cx<T> *__restrict__ I, long sI, long fs, long ip, int src,
                                                   int dst0, int ndst, int dtop, double scale, KTab kt,
-                                                  const cx<T> *__restrict__ twx)
+                                                  const cx<T> *__restrict__ twx, int spec, int nrows)
 {
     constexpr int NX = 1 << LOGNX, E = NX / 64;
     using L = Lds<T, NX>;
@@ -1716,7 +1742,7 @@ __global__ __launch_bounds__(NT) void k_xfwdinv_w(cx<T> *__restrict__ I, long sI
     __shared__ cx<T> lds[XW * L::PITCH];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int ky = blockIdx.x * XW + w;
-    if (ky >= kt.nk) return;
+    if (ky >= nrows) return;
     cx<T> *row = lds + w * L::PITCH;
     const int m = blockIdx.z;
     const long r0 = (long)ky * NX, rI = (long)ky * ip;  // tanh table row, x-intermediate row
@@ -1740,6 +1766,17 @@ __global__ __launch_bounds__(NT) void k_xfwdinv_w(cx<T> *__restrict__ I, long sI
         for (int i = 0; i < E; ++i) kz_th<R>(kxv[i], kyv, h, thd[i], kz[i], th[i]);
     }
     wave_fft<T, LOGNX, false, true>(s, row, twx, lane);
+    if (spec) {
+        // the one output, d phi/dz (ndst = 1), left in x-spectral space for
+        // the RK epilogue
+        cx<T> *out = base + dtop * fs + rI;
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const T mul = T(R(scale) * kz[i] * th[i]);
+            out[lane + 64 * i] = {mul * s[i].x, mul * s[i].y};
+        }
+        return;
+    }
     R kj[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) kj[i] = R(scale);

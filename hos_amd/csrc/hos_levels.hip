@@ -167,6 +167,9 @@ static int ydiag_t(Eng<T> &e)
         rows += da.nzr[k - 1 + DA::ETA];
     }
     da.dst = 0;
+    // the last pass's phi^(M) feeds only d phi^(M)/dz, which stays in x-spectral
+    // space on the rows the mask keeps (Eng::diag_levels)
+    da.nzo = LV == c->M - 1 ? c->my : c->Nk;
     da.wf = c->ix.w[LV];
     da.etaf = c->ix.eta;
     const int tx = c->tx[HOS_TILE_PAIR];
@@ -179,7 +182,7 @@ static int ydiag_t(Eng<T> &e)
             // eta read (pass 1: written) and W^(L) written (physical), the
             // tiles read, phi^(L+1) written (spectral rows)
             Prof pf(c, e.S(), HOS_K_YDIAG,
-                    (double)e.nm * sizeof(T) * c->p.Nx * (2.0 * c->p.Ny + 2.0 * rows + 2.0 * c->Nk));
+                    (double)e.nm * sizeof(T) * c->p.Nx * (2.0 * c->p.Ny + 2.0 * rows + 2.0 * da.nzo));
             hipLaunchKernelGGL((k_ydiag<T, LG, TXP, LV>), grid, dim3(NT_DIAG), 0, e.S(), e.phys(), e.P(), c->ix,
                                (int)e.comp(), e.I(), e.sI(), c->fsI, da, (int)c->ip, e.twy());
         })
