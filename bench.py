@@ -377,7 +377,10 @@ def main():
                         "instance (seed 1)",
             "members": args.members, "graph": graph, "y_tiles_c2r_r2c_pair": f32["tiles"],
             "x_inverse_passes": f32["xpass"], "member_groups": f32["groups"],
-            "rhs_fft_plan": {1: "16 8 8", 2: "8 8 4 4", 3: "4 4 4 4 4", 4: "16 16 4"}.get(f32["plan"], f32["plan"]),
+            # the plan family of the RHS pass's row FFTs (fft_stage_bits); the built-in
+            # rule gives 16 8 8 at 512 threads, 16 16 4 at 256 (the FP32 RHS pass over W fields)
+            "rhs_fft_plan": {1: "built-in rule", 2: "radix <= 8", 3: "radix 4", 4: "largest radix"}.get(f32["plan"],
+                                                                                                    f32["plan"]),
             "w_expansion": "diagonal passes" if f32["diag"] else "level passes",
             "fused_level_passes": f32["fuse"],
             "offset_member_groups": f32["skew"],
