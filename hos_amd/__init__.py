@@ -47,7 +47,7 @@ EXPORTS = [
     "hos_autotune", "hos_copy_bandwidth", "hos_get_tile", "hos_query", "hos_rhs_noalloc",
 ]
 (HOS_TILE_C2R, HOS_TILE_R2C, HOS_TILE_PAIR, HOS_TILE_ALL, HOS_TILE_XPASS, HOS_TILE_GROUPS, HOS_TILE_BAND,
- HOS_TILE_PLAN, HOS_TILE_FUSE, HOS_TILE_SKEW, HOS_TILE_DIAG) = (0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10)
+ HOS_TILE_PLAN, HOS_TILE_FUSE, HOS_TILE_SKEW, HOS_TILE_DIAG, HOS_TILE_XBLOCK) = (0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11)
 HOS_OP_DX, HOS_OP_DY, HOS_OP_DZ, HOS_OP_SUM, HOS_OP_DEALIAS = 1, 2, 3, 4, 5
 # include/hos_ic.h
 IC_EXPORTS = ["hos_ic_jonswap", "hos_ic_uniform", "hos_ic_peak_period", "hos_dealias_state"]

@@ -36,6 +36,7 @@ struct hos_ctx {
     int plan = 0;           // FFT radix plan of the RHS pass (fft_stage_bits; plans 1-3 need Ny >= 512)
     int fuse = 1;           // a W level's y c2r carries the next level's y r2c (k_yc2r_phi)
     int diag = 0;           // W levels as diagonal passes (k_ydiag; HOS_TILE_DIAG)
+    int xblock = 1;         // blocked x-intermediates in the diagonal form with wave x passes (HOS_TILE_XBLOCK)
     // member groups: with G > 1, the active members split into G nearly equal
     // ranges that run as independent launch chains on their own streams (gst),
     // forked from and joined to the context stream each step
@@ -48,7 +49,7 @@ struct hos_ctx {
     bool f32 = false;
     long fs = 0;    // complex elements per spectral field  (Nk*Nx)
     long ip = 0;    // row pitch of the x-intermediates I    (Nx + 16)
-    long fsI = 0;   // complex elements per x-intermediate   (Nk*ip)
+    long fsI = 0;   // complex elements per x-intermediate   (xi_rows(Nk)*ip)
     long fsP = 0;   // real elements per physical field     (Nx*Ny + one spare row)
     hipStream_t stream = nullptr;
     // device memory
@@ -230,6 +231,10 @@ struct Eng {
     bool band = false;       // the stage input is a masked state: skip its masked modes
     hipEvent_t mark = nullptr;  // recorded on S() at point mark_at of the RHS (levels 0..M-1, M: after them)
     int mark_at = -1;
+    // the x-intermediates of the RHS under way are blocked (xi_at): set by
+    // rhs() for the diagonal form with wave x passes, which is the whole
+    // chain of writers and readers of those slots
+    int xb = 0;
     hipStream_t S() const { return st ? st : c->stream; }
     void mark_if(int at) const
     {
@@ -304,12 +309,17 @@ struct Eng {
             if constexpr (x_wave<T>(LG)) {
                 if (c->xwave) {
                     dim3 gw((nz + XW - 1) / XW, ngroups, nm);
-                    hipLaunchKernelGGL((k_xinv_w<T, LG>), gw, dim3(NT), 0, S(), Y, sY(), I(), sI(), c->fs, c->fsI, c->ip, ops,
-                                       kt(), twx(), nz, cmx());
+                    if (xb)
+                        hipLaunchKernelGGL((k_xinv_w<T, LG, true>), gw, dim3(NT), 0, S(), Y, sY(), I(), sI(), c->fs, c->fsI,
+                                           c->ip, ops, kt(), twx(), nz, cmx());
+                    else
+                        hipLaunchKernelGGL((k_xinv_w<T, LG, false>), gw, dim3(NT), 0, S(), Y, sY(), I(), sI(), c->fs, c->fsI,
+                                           c->ip, ops, kt(), twx(), nz, cmx());
                     waved = true;
                 }
             }
             if (!waved) {
+                if (xb) return fail(HOS_E_INVALID, "blocked x-intermediates need the wave x passes");
                 hipLaunchKernelGGL((k_xinv<T, LG, ROWS>), grid, dim3(NT), 0, S(), Y, sY(), I(), sI(), c->fs, c->fsI, c->ip,
                                    ops, kt(), twx(), nz, cmx());
             }
@@ -377,12 +387,17 @@ struct Eng {
             if constexpr (x_wave<T>(LG)) {
                 if (c->xwave) {
                     dim3 gw((nrows + XW - 1) / XW, 1, nm);
-                    hipLaunchKernelGGL((k_xfwdinv_w<T, LG>), gw, dim3(NT), 0, S(), I(), sI(), c->fsI, c->ip, src, dst0,
-                                       ndst, dtop, scale, kt(), twx(), (int)spec, nrows);
+                    if (xb)
+                        hipLaunchKernelGGL((k_xfwdinv_w<T, LG, true>), gw, dim3(NT), 0, S(), I(), sI(), c->fsI, c->ip, src,
+                                           dst0, ndst, dtop, scale, kt(), twx(), (int)spec, nrows);
+                    else
+                        hipLaunchKernelGGL((k_xfwdinv_w<T, LG, false>), gw, dim3(NT), 0, S(), I(), sI(), c->fsI, c->ip, src,
+                                           dst0, ndst, dtop, scale, kt(), twx(), (int)spec, nrows);
                     waved = true;
                 }
             }
             if (!waved) {
+                if (xb) return fail(HOS_E_INVALID, "blocked x-intermediates need the wave x passes");
                 hipLaunchKernelGGL((k_xfwdinv<T, LG, ROWS>), grid, dim3(NT), 0, S(), I(), sI(), c->fsI, c->ip, src, dst0,
                                    ndst, dtop, scale, kt(), twx(), (int)spec, nrows);
             }
@@ -583,11 +598,18 @@ struct Eng {
     // behind the one field it reads.)
     int rhs(const cx<double> *Y, const RkArgs &rk)
     {
+        // blocked x-intermediates: the diagonal form's x passes (wave form)
+        // write them, its y passes (k_ydiag, the RHS pass) read them and
+        // k_ydiag's r2c output is read back by the wave x pass; the RHS
+        // pass's output and the spectral top-order term stay row by row
+        xb = c->xblock && c->diag && c->M >= 2 && c->xwave && x_wave<T>(c->lognx) ? 1 : 0;
         int rc = c->diag ? diag_levels(Y) : w_levels(Y, true);
-        if (rc) return rc;
-        mark_if(c->M);
-        rc = rhs_pair();
+        if (!rc) {
+            mark_if(c->M);
+            rc = rhs_pair();
+        }
         if (!rc) rc = xfwd_rhs(0, -1, Y, rk, c->diag && c->M >= 2 ? dslot_w(0) : -1);
+        xb = 0;
         return rc;
     }
 
@@ -609,6 +631,7 @@ struct Eng {
             drows += dfr.nzr[f];
         }
         dfr.nzo = band ? c->my : c->Nk;
+        dfr.blk = xb;
         // eta and d^j phi^(a), a + j <= M (or W^(1) .. W^(M-1)), from the physical grid
         const int nin = 1 + (WF ? MO - 1 : (MO - 1) * MO / 2);
         const int tx = c->tx[HOS_TILE_PAIR];

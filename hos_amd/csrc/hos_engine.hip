@@ -196,7 +196,7 @@ static void layout(hos_ctx *c)
     // tile reads 3.33 -> 3.58 TB/s; y c2r 41.7 -> 39.1 us, FP64 plain step
     // 1.126 -> 1.114 ms, the two-group FP32 batch unchanged)
     c->ip = p.Nx + 16;
-    c->fsI = (long)c->Nk * c->ip;
+    c->fsI = (long)xi_rows(c->Nk) * c->ip;  // whole blocks of rows (xi_at)
     // FP32: one spare row per physical field.  Fields a power of two apart (4 MB
     // at 1024^2) put the same point of every field a pass reads on the same
     // HBM channels; one row off, the level r2c passes went from 33 to 31 us
@@ -902,6 +902,13 @@ extern "C" int hos_set_tile(hos_ctx *c, int kind, int tx)
         drop_graphs(c);
         return HOS_OK;
     }
+    if (kind == HOS_TILE_XBLOCK) {
+        if (tx < 0 || tx > 2) return fail(HOS_E_INVALID, "blocked x-intermediates %d: 0 default, 1 off, 2 on", tx);
+        if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
+        c->xblock = tx != 1;
+        drop_graphs(c);
+        return HOS_OK;
+    }
     if (kind == HOS_TILE_SKEW) {
         if (tx < 0 || tx > 2) return fail(HOS_E_INVALID, "offset member groups %d: 0 default, 1 off, 2 on", tx);
         if (c->async_pending) return fail(HOS_E_STATE, "step in flight");
@@ -946,6 +953,7 @@ extern "C" int hos_get_tile(hos_ctx *c, int kind)
     if (kind == HOS_TILE_FUSE) return c->fuse ? 2 : 1;
     if (kind == HOS_TILE_SKEW) return c->skew ? 2 : 1;
     if (kind == HOS_TILE_DIAG) return c->diag ? 2 : 1;
+    if (kind == HOS_TILE_XBLOCK) return c->xblock ? 2 : 1;
     if (kind == HOS_TILE_PLAN) return c->plan + 1;
     if (kind == HOS_TILE_GROUPS) return c->ngroups;
     if (kind < HOS_TILE_C2R || kind > HOS_TILE_PAIR) return fail(HOS_E_INVALID, "unknown tile kind %d", kind);

@@ -82,6 +82,18 @@ __device__ __forceinline__ void store_nt(cx<T> *p, cx<T> v)
     }
 }
 
+// two adjacent complex values (16-byte aligned address) with the non-temporal hint
+template <class T>
+__device__ __forceinline__ void store2_nt(cx<T> *p, cx<T> a, cx<T> b)
+{
+    if constexpr (sizeof(T) == 4) {
+        __builtin_nontemporal_store(vec_t<float>{a.x, a.y, b.x, b.y}, reinterpret_cast<vec_t<float> *>(p));
+    } else {
+        store_nt(p, a);
+        store_nt(p + 1, b);
+    }
+}
+
 // loads with the non-temporal hint, for data at its last use in a pass whose
 // working set is far larger than the caches: the RK epilogue's reads of the
 // RHS pair, the stage input and the accumulator when many rows are in flight
@@ -206,6 +218,24 @@ struct XGeom {
 // columns the mask keeps.  The modes it skips are exact zeros, which the
 // transforms would only carry along, so results are unchanged.
 __device__ __forceinline__ bool col_kept(int kx, int nx, int cmx) { return cmx == 0 || kx < cmx || kx > nx - cmx; }
+
+// Blocked x-intermediates (blk): the rows ky in blocks of XKB, and inside a
+// block the rows of each column pair x = 2c, 2c+1 side by side: element
+// (ky, x) of the block starting at row b sits at b ip + c 2 XKB + (ky - b) 2
+// + (x & 1).  A block occupies the XKB ip elements its rows occupy row by row
+// (a field holds ceil(Nk / XKB) XKB rows, xi_rows).  A y pass reads or writes
+// one column pair of every row: 16 bytes per row row by row, XKB*16
+// contiguous bytes per block blocked (a quarter of the cache-line requests).
+// The wave x passes, XKB rows per workgroup, store and load a block through
+// LDS as one contiguous run.  Otherwise (ky ip + x) the rows lie one by one.
+constexpr int XKB = 4;
+constexpr int xi_rows(int nk) { return (nk + XKB - 1) / XKB * XKB; }
+__device__ __forceinline__ long xi_at(int ky, int x, long ip, int blk)
+{
+    if (!blk) return (long)ky * ip + x;
+    const int b = ky & ~(XKB - 1);
+    return (long)b * ip + (long)(x >> 1) * (2 * XKB) + (ky - b) * 2 + (x & 1);
+}
 
 // x inverse pass: spectral (FP64 stage input, with multiplier) -> I.  The
 // source row is read once into registers (FP64) and every output of the
@@ -663,7 +693,8 @@ struct ProZvel {
 // (x0+2p, x0+2p+1 side by side); 16 consecutive ky of one pair per 16 lanes
 // (contiguous in LDS)
 template <class T, int LOGNY, int NPR, int NTT = NT>
-__device__ __forceinline__ void r2c_store(const cx<T> *buf, cx<T> *__restrict__ out, int x0, int nx, int nzo = -1)
+__device__ __forceinline__ void r2c_store(const cx<T> *buf, cx<T> *__restrict__ out, int x0, int nx, int nzo = -1,
+                                          int blk = 0)
 {
     constexpr int NY = 1 << LOGNY, NK = NY / 2 + 1;
     constexpr int NO = ((NK + 15) / 16) * 16 * NPR;
@@ -673,7 +704,7 @@ __device__ __forceinline__ void r2c_store(const cx<T> *buf, cx<T> *__restrict__ 
         if (ky >= nzo) continue;  // nzo <= NK
         const cx<T> zk = buf[Lds<T, NY>::at(p, ky)];
         const cx<T> zm = buf[Lds<T, NY>::at(p, (NY - ky) & (NY - 1))];
-        store2(out + (long)ky * nx + x0 + 2 * p, cx<T>{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)},
+        store2(out + xi_at(ky, x0 + 2 * p, nx, blk), cx<T>{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)},
                cx<T>{T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x)});
     }
 }
@@ -854,6 +885,7 @@ struct RhsDeferred {
     int slot[M + 4];  // x-intermediate slot of eta_x, eta_y, phi_x, phi_y, then W^(M) term k = 0..M-1
     int nzr[M + 4];   // rows that may be nonzero (a band of a masked state, or Nk)
     int nzo;          // output rows written (my: the final x pass masks the rest)
+    int blk;          // the x-intermediates are blocked (xi_at)
 };
 
 // y c2r tile loads of NF fields at once (x-rows x0 .. x0+2 NPR-1 of each,
@@ -867,7 +899,7 @@ struct C2rBatch {
     static constexpr int NL = ((NK + 15) / 16) * 16 * NPR, NLB = NF * NL, PERL = (NLB + NTT - 1) / NTT;
     cx<T> A[PERL], B[PERL];
     __device__ __forceinline__ void load(const cx<T> *__restrict__ base, long fs, const int *slot, const int *nzr,
-                                         int x0, int nx)
+                                         int x0, int nx, int blk)
     {
         const int tid = opaque_tid<NTT>();
 #pragma unroll
@@ -884,7 +916,7 @@ struct C2rBatch {
                     nz = nzr[k];
                 }
             if (ky < nz) {
-                load2(base + sl * fs + (long)ky * nx + x0 + 2 * p, A[q], B[q]);
+                load2(base + sl * fs + xi_at(ky, x0 + 2 * p, nx, blk), A[q], B[q]);
             } else {  // a row of a masked state's band that is zero
                 A[q] = {T(0), T(0)};
                 B[q] = {T(0), T(0)};
@@ -1078,9 +1110,9 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
 #pragma unroll
         for (int q = 0; q < PER; ++q) ps[q].load(ph, ix, m, x0 + tp[q], ty[q]);
         if (NB > 1)
-            tb.load(Im, fsI, dfr.slot, dfr.nzr, x0, nx);
+            tb.load(Im, fsI, dfr.slot, dfr.nzr, x0, nx, dfr.blk);
         else
-            tl.load(Im, fsI, dfr.slot, dfr.nzr, x0, nx);
+            tl.load(Im, fsI, dfr.slot, dfr.nzr, x0, nx, dfr.blk);
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int e = threadIdx.x + q * NTT;
@@ -1095,9 +1127,9 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
         }
     } else {
         if (NB > 1)
-            tb.load(Im, fsI, dfr.slot, dfr.nzr, x0, nx);
+            tb.load(Im, fsI, dfr.slot, dfr.nzr, x0, nx, dfr.blk);
         else
-            tl.load(Im, fsI, dfr.slot, dfr.nzr, x0, nx);
+            tl.load(Im, fsI, dfr.slot, dfr.nzr, x0, nx, dfr.blk);
     }
     tws.store(twl);
     // phase A: y c2r of the deferred fields, BF at a time (BF*NPR rows share
@@ -1113,9 +1145,9 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
         else
             tl.fill(buf);
         if (b + 2 < NB)
-            tb.load(Im, fsI, dfr.slot + (b + 1) * BF, dfr.nzr + (b + 1) * BF, x0, nx);
+            tb.load(Im, fsI, dfr.slot + (b + 1) * BF, dfr.nzr + (b + 1) * BF, x0, nx, dfr.blk);
         else if (b + 1 < NB)
-            tl.load(Im, fsI, dfr.slot + (b + 1) * BF, dfr.nzr + (b + 1) * BF, x0, nx);
+            tl.load(Im, fsI, dfr.slot + (b + 1) * BF, dfr.nzr + (b + 1) * BF, x0, nx, dfr.blk);
         if (nf == 3)
             lds_fft<T, LOGNY, 3 * NPR, NTT, true, PLAN>(buf, twl);
         else if (nf == 2)
@@ -1240,6 +1272,7 @@ struct DiagArgs {
     int nzo;       // rows of that r2c written (the last pass: the rows the mask keeps)
     int wf;        // physical field of W^(L)
     int etaf;      // physical field of eta (pass 1)
+    int blk;       // the x-intermediates are blocked (xi_at)
 };
 
 // 256-thread workgroups: 53.7 against 56.6 us per pass at 8 x 1024^2, M = 5
@@ -1272,9 +1305,9 @@ __device__ __forceinline__ void diag_tile(const Phys<T> &ph, T *__restrict__ P, 
     TwLds<T, NY, NTT> tws;
     tws.load(twy);
     if (NB > 1)
-        tb.load(Im, fsI, da.slot, da.nzr, x0, nx);
+        tb.load(Im, fsI, da.slot, da.nzr, x0, nx, da.blk);
     else
-        tl.load(Im, fsI, da.slot, da.nzr, x0, nx);
+        tl.load(Im, fsI, da.slot, da.nzr, x0, nx, da.blk);
     F eta[PER], ek[PER];
     KSum<F, C> ps[PER], ws[PER];  // the sums of phi^(L+1) (negated last) and W^(L)
 #pragma unroll
@@ -1291,9 +1324,9 @@ __device__ __forceinline__ void diag_tile(const Phys<T> &ph, T *__restrict__ P, 
         else
             tl.fill(buf);
         if (b + 2 < NB)
-            tb.load(Im, fsI, da.slot + (b + 1) * BF, da.nzr + (b + 1) * BF, x0, nx);
+            tb.load(Im, fsI, da.slot + (b + 1) * BF, da.nzr + (b + 1) * BF, x0, nx, da.blk);
         else if (b + 1 < NB)
-            tl.load(Im, fsI, da.slot + (b + 1) * BF, da.nzr + (b + 1) * BF, x0, nx);
+            tl.load(Im, fsI, da.slot + (b + 1) * BF, da.nzr + (b + 1) * BF, x0, nx, da.blk);
         if (nf == 3)
             lds_fft<T, LOGNY, 3 * NPR, NTT, true>(buf, twl);
         else if (nf == 2)
@@ -1342,7 +1375,7 @@ __device__ __forceinline__ void diag_tile(const Phys<T> &ph, T *__restrict__ P, 
         }
     }
     lds_fft<T, LOGNY, NPR, NTT, false>(buf, twl);
-    r2c_store<T, LOGNY, NPR, NTT>(buf, out, x0, nx, da.nzo);
+    r2c_store<T, LOGNY, NPR, NTT>(buf, out, x0, nx, da.nzo, da.blk);
 }
 
 template <class T, int LOGNY, int TXP, int LV>
@@ -1655,10 +1688,79 @@ __global__ __launch_bounds__(NT) void k_xfwd_rhs(const cx<T> *__restrict__ I, lo
 // wave_fft transforms in registers, so the loads and stores of a row are
 // coalesced 64-lane accesses and no workgroup barrier is needed.  A workgroup
 // is XW waves on consecutive rows; grid (ceil(Nk/XW), groups, members); a wave
-// past row Nk-1 returns at once.
+// past row Nk-1 returns at once, unless the x-intermediates are blocked
+// (xi_at): then a workgroup's XW = XKB rows are one block, which the
+// workgroup stores and loads together (xblock_store, xblock_load), and every
+// wave takes part (a wave past the last row transforms data it never stores).
 constexpr int XW = NT / 64;
+static_assert(XW == XKB, "a wave x-pass workgroup holds one block of rows");
 
+// units of 16 bytes (one column pair of one row) per thread in a block
+template <int LOGNX>
+constexpr int xblock_units() { return XKB * (1 << LOGNX) / 2 / NT; }
+
+
+// the workgroup's rows ky0 .. ky0+XKB-1 from the waves' registers (v, this
+// wave's row) to the blocked field dst: each wave puts its row in its LDS
+// row, then consecutive threads store consecutive 16-byte units of the
+// block (unit u = c XKB + r: column pair c of row ky0+r); rows >= nrows are
+// left out.  Called by every wave of the workgroup (two barriers).
 template <class T, int LOGNX>
+__device__ __forceinline__ void xblock_store(const cx<T> *v, cx<T> *row, const cx<T> *lds, cx<T> *__restrict__ dst,
+                                             int ky0, int nrows, long ip, int lane)
+{
+    constexpr int NX = 1 << LOGNX, E = NX / 64, NU = xblock_units<LOGNX>();
+    using L = Lds<T, NX>;
+    wave_sync();  // the FFT's last reads of the row precede these writes
+#pragma unroll
+    for (int i = 0; i < E; ++i) row[L::at(0, lane + 64 * i)] = v[i];
+    __syncthreads();
+    const int nv = min(XKB, nrows - ky0);
+    cx<T> *b = dst + (long)ky0 * ip;
+#pragma unroll 2
+    for (int q = 0; q < NU; ++q) {
+        const int u = threadIdx.x + q * NT, c = u / XKB, r = u % XKB;
+        if (r < nv) {
+            const cx<T> *rr = lds + r * L::PITCH;
+            store2_nt(b + 2 * u, rr[L::at(0, 2 * c)], rr[L::at(0, 2 * c + 1)]);
+        }
+    }
+    __syncthreads();  // the rows are free for the next transform
+}
+
+// the reverse: the block of rows ky0 .. ky0+XKB-1 of the blocked field src
+// (rows < nrows) through the LDS rows into each wave's registers s (natural
+// order, lane l holding x = l + 64 i).  Called by every wave of the workgroup.
+template <class T, int LOGNX>
+__device__ __forceinline__ void xblock_load(cx<T> *s, cx<T> *row, cx<T> *lds, const cx<T> *__restrict__ src, int ky0,
+                                            int nrows, long ip, int lane)
+{
+    constexpr int NX = 1 << LOGNX, E = NX / 64, NU = xblock_units<LOGNX>();
+    using L = Lds<T, NX>;
+    const int nv = min(XKB, nrows - ky0);
+    const cx<T> *b = src + (long)ky0 * ip;
+    cx<T> a[NU], bb[NU];
+#pragma unroll
+    for (int q = 0; q < NU; ++q) {
+        const int u = threadIdx.x + q * NT;
+        if (u % XKB < nv) load2(b + 2 * u, a[q], bb[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < NU; ++q) {
+        const int u = threadIdx.x + q * NT, c = u / XKB, r = u % XKB;
+        if (r < nv) {
+            cx<T> *rr = lds + r * L::PITCH;
+            rr[L::at(0, 2 * c)] = a[q];
+            rr[L::at(0, 2 * c + 1)] = bb[q];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < E; ++i) s[i] = row[L::at(0, lane + 64 * i)];
+    wave_sync();  // read before the FFT's exchanges write the row
+}
+
+template <class T, int LOGNX, bool BLK>
 __global__ __launch_bounds__(NT) void k_xinv_w(const cx<double> *__restrict__ Y, long sY, cx<T> *__restrict__ I,
                                                long sI, long fs, long fsI, long ip, XinvOps ops, KTab kt,
                                                const cx<T> *__restrict__ twx, int nzr, int cmx)
@@ -1669,7 +1771,8 @@ __global__ __launch_bounds__(NT) void k_xinv_w(const cx<double> *__restrict__ Y,
     __shared__ cx<T> lds[XW * L::PITCH];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int ky = blockIdx.x * XW + w;
-    if (ky >= nzr) return;  // rows of the band only (col_kept)
+    const bool live = ky < nzr;  // rows of the band only (col_kept)
+    if (!BLK && !live) return;
     cx<T> *row = lds + w * L::PITCH;
     const int grp = blockIdx.y, m = blockIdx.z;
     const long r0 = (long)ky * NX;
@@ -1684,7 +1787,7 @@ __global__ __launch_bounds__(NT) void k_xinv_w(const cx<double> *__restrict__ Y,
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             const int kx = lane + 64 * i;
-            const bool keep = col_kept(kx, NX, cmx);
+            const bool keep = live && col_kept(kx, NX, cmx);
             sd[i] = keep ? src[kx] : cx<double>{0.0, 0.0};
             kxv[i] = kt.kx[kx];
             if constexpr (sizeof(R) == 8) thd[i] = keep ? tht[kx] : 1.0;
@@ -1723,15 +1826,19 @@ __global__ __launch_bounds__(NT) void k_xinv_w(const cx<double> *__restrict__ Y,
             for (int i = 0; i < E; ++i) v[i] = {T(sr[i].x), T(sr[i].y)};
         }
         wave_fft<T, LOGNX, true>(v, row, twx, lane);
-        cx<T> *dst = I + m * sI + op.dst * fsI + (long)ky * ip;
+        cx<T> *dst = I + m * sI + op.dst * fsI;
+        if constexpr (BLK) {
+            xblock_store<T, LOGNX>(v, row, lds, dst, ky - w, nzr, ip, lane);
+        } else {
 #pragma unroll
-        for (int i = 0; i < E; ++i) store_nt(dst + lane + 64 * i, v[i]);
+            for (int i = 0; i < E; ++i) store_nt(dst + (long)ky * ip + lane + 64 * i, v[i]);
+        }
     }
 }
 
 // x forward pass of phi^(lev+1), d^j/dz^j for j = 1..ndst and the x inverse of
 // each, one row per wave: the spectral row stays in registers throughout
-template <class T, int LOGNX>
+template <class T, int LOGNX, bool BLK>
 __global__ __launch_bounds__(NT) void k_xfwdinv_w(cx<T> *__restrict__ I, long sI, long fs, long ip, int src,
                                                   int dst0, int ndst, int dtop, double scale, KTab kt,
                                                   const cx<T> *__restrict__ twx, int spec, int nrows)
@@ -1742,7 +1849,8 @@ __global__ __launch_bounds__(NT) void k_xfwdinv_w(cx<T> *__restrict__ I, long sI
     __shared__ cx<T> lds[XW * L::PITCH];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int ky = blockIdx.x * XW + w;
-    if (ky >= nrows) return;
+    const bool live = ky < nrows;
+    if (!BLK && !live) return;
     cx<T> *row = lds + w * L::PITCH;
     const int m = blockIdx.z;
     const long r0 = (long)ky * NX, rI = (long)ky * ip;  // tanh table row, x-intermediate row
@@ -1752,14 +1860,15 @@ __global__ __launch_bounds__(NT) void k_xfwdinv_w(cx<T> *__restrict__ I, long sI
     const double h = kt.h(m), kyv = ky * kt.ky0;
     cx<T> s[E];
     R kz[E], th[E];
+    if constexpr (BLK) xblock_load<T, LOGNX>(s, row, lds, base + src * fs, ky - w, nrows, ip, lane);
     {
         double kxv[E], thd[E];
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             const int kx = lane + 64 * i;
-            s[i] = in[kx];
+            if constexpr (!BLK) s[i] = in[kx];
             kxv[i] = kt.kx[kx];
-            if constexpr (sizeof(R) == 8) thd[i] = tht[kx];
+            if constexpr (sizeof(R) == 8) thd[i] = live ? tht[kx] : 1.0;
             else thd[i] = 1.0;
         }
 #pragma unroll
@@ -1768,7 +1877,8 @@ __global__ __launch_bounds__(NT) void k_xfwdinv_w(cx<T> *__restrict__ I, long sI
     wave_fft<T, LOGNX, false, true>(s, row, twx, lane);
     if (spec) {
         // the one output, d phi/dz (ndst = 1), left in x-spectral space for
-        // the RK epilogue
+        // the RK epilogue (row by row, as that pass reads it)
+        if (!live) return;
         cx<T> *out = base + dtop * fs + rI;
 #pragma unroll
         for (int i = 0; i < E; ++i) {
@@ -1789,9 +1899,13 @@ __global__ __launch_bounds__(NT) void k_xfwdinv_w(cx<T> *__restrict__ I, long sI
             v[i] = {mul * s[i].x, mul * s[i].y};
         }
         wave_fft<T, LOGNX, true, true>(v, row, twx, lane);
-        cx<T> *out = base + ((j == ndst && dtop >= 0) ? dtop : dst0 + j - 1) * fs + rI;
+        cx<T> *out = base + ((j == ndst && dtop >= 0) ? dtop : dst0 + j - 1) * fs;
+        if constexpr (BLK) {
+            xblock_store<T, LOGNX>(v, row, lds, out, ky - w, nrows, ip, lane);
+        } else {
 #pragma unroll
-        for (int i = 0; i < E; ++i) store_nt(out + lane + 64 * i, v[i]);
+            for (int i = 0; i < E; ++i) store_nt(out + rI + lane + 64 * i, v[i]);
+        }
     }
 }
 

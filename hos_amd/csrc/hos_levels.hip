@@ -172,6 +172,7 @@ static int ydiag_t(Eng<T> &e)
     da.nzo = LV == c->M - 1 ? c->my : c->Nk;
     da.wf = c->ix.w[LV];
     da.etaf = c->ix.eta;
+    da.blk = e.xb;
     const int tx = c->tx[HOS_TILE_PAIR];
     LOGN_SWITCH(c->logny, {
         TX_SWITCH(T, tx, {

@@ -280,9 +280,20 @@ int hos_use_graph(hos_ctx *c, int enable);
  * instead of the M(M-1)/2 derivative fields.  No derivative field reaches
  * the physical grid.  Results agree with the mode off to rounding. */
 #define HOS_TILE_DIAG 10
+/* HOS_TILE_XBLOCK: 1 off, 2 on (0: default, on).  On, in the diagonal form
+ * with row-per-wave x passes (HOS_TILE_DIAG and HOS_TILE_XPASS both 2, M >= 2),
+ * the x-intermediates between the x and y passes are stored in blocks of 4
+ * rows inside which the rows of each column pair lie side by side: the y
+ * passes, which read and write one column pair of every row, then move 64
+ * contiguous bytes per block instead of 16 bytes per row, and the x passes
+ * store and load a block through LDS as one contiguous run.  Only where the
+ * values lie changes: results are the same bits.  hos_autotune leaves it as
+ * set. */
+#define HOS_TILE_XBLOCK 11
 int hos_set_tile(hos_ctx *c, int kind, int tx);
 /* current setting of one kind (0-2: tx; HOS_TILE_XPASS: 1 or 2; HOS_TILE_GROUPS: 1..4;
- * HOS_TILE_BAND, HOS_TILE_FUSE, HOS_TILE_SKEW, HOS_TILE_DIAG: 1 or 2; HOS_TILE_PLAN: 1..4),
+ * HOS_TILE_BAND, HOS_TILE_FUSE, HOS_TILE_SKEW, HOS_TILE_DIAG, HOS_TILE_XBLOCK: 1 or 2;
+ * HOS_TILE_PLAN: 1..4),
  * < 0 on error */
 int hos_get_tile(hos_ctx *c, int kind);
 

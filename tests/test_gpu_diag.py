@@ -180,3 +180,55 @@ def test_diag_large_batch_matches_mode_off(hos):
         err = rel_l2(b, a)
         print(f"  rel L2 {err:.2e}", flush=True)
         assert err < 1e-6, err
+
+
+@pytest.mark.parametrize("Nx,Ny", [(512, 64), (1024, 32), (512, 512)])
+@pytest.mark.parametrize("nlevs", [1, 2, 4])
+@pytest.mark.parametrize("band", [1, 2])
+def test_xblock_same_bits(hos, Nx, Ny, nlevs, band):
+    """Blocked x-intermediates (HOS_TILE_XBLOCK) change only where the values
+    lie: with the diagonal passes and one row per wave in the x passes, RK4
+    steps of three members over two member groups, and one RHS, are the same
+    bits with the blocks on and off.  Ny = 32 and 64: Nk = 17 and 33 rows, so
+    the last block holds one row (and the band of a masked state ends inside
+    a block)."""
+    print(f"xblock {Nx}x{Ny} nlevs={nlevs} band={band}", flush=True)
+    Lx, Ly = 2 * np.pi * Nx / 64, 2 * np.pi * Ny / 64
+    nm = 3
+    states = [random_state(Nx, Ny, nlevs, amp=0.05, slope=3.0, seed=110 + i, Lx=Lx, Ly=Ly) for i in range(nm)]
+    out = {}
+    for xb in (1, 2):
+        s = solver(hos, Nx, Ny, nlevs, 1, nm=nm, Lx=Lx, Ly=Ly, depth=1.1)
+        s.set_tile(2, hos.HOS_TILE_XPASS)
+        s.set_tile(2, hos.HOS_TILE_GROUPS)
+        s.set_tile(band, hos.HOS_TILE_BAND)
+        s.set_tile(xb, hos.HOS_TILE_XBLOCK)
+        assert s.get_tile(hos.HOS_TILE_XBLOCK) == xb
+        r = s.rhs(states[0])
+        for i, u in enumerate(states):
+            s.set_state(u, i)
+        s.step(3, 0.05)
+        out[xb] = (r, [s.get_state(i) for i in range(nm)], [s.energy(i) for i in range(nm)])
+        s.close()
+    assert np.array_equal(out[1][0], out[2][0])
+    for a, b in zip(out[1][1], out[2][1]):
+        assert np.array_equal(a, b)
+    assert out[1][2] == out[2][2]
+
+
+@pytest.mark.parametrize("Nx,Ny", [(512, 64), (1024, 128)])
+def test_xblock_rhs_vs_oracle(hos, oracle_mod, Nx, Ny):
+    """The FP32 diagonal form with wave x passes and blocked x-intermediates
+    against the FP64 oracle, finite depth, ramp on."""
+    nlevs, depth = 4, 0.9
+    Lx, Ly = 2 * np.pi * Nx / 64, 2 * np.pi * Ny / 64
+    print(f"xblock rhs {Nx}x{Ny}", flush=True)
+    o = oracle_mod.Oracle(Nx, Ny, nlevs, Lx, Ly, depth=depth, rampflg=1, Tramp=2.0)
+    s = solver(hos, Nx, Ny, nlevs, 1, Lx=Lx, Ly=Ly, depth=depth, rampflg=1, Tramp=2.0)
+    s.set_tile(2, hos.HOS_TILE_XPASS)
+    s.set_tile(2, hos.HOS_TILE_XBLOCK)
+    u = random_state(Nx, Ny, nlevs, amp=0.05, slope=3.0, seed=31, Lx=Lx, Ly=Ly)
+    err = rel_l2(s.rhs(u, 1.3), o.rhs_hos(u, 1.3))
+    print(f"  rel L2 {err:.2e}", flush=True)
+    assert err < TOL32, err
+    s.close()
