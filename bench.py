@@ -78,6 +78,8 @@ def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_s
             s.set_tile(tiles[7], hos.HOS_TILE_SKEW)
         if len(tiles) > 8:
             s.set_tile(tiles[8], hos.HOS_TILE_DIAG)
+        if len(tiles) > 9:
+            s.set_tile(tiles[9], hos.HOS_TILE_XBLOCK)
         tiles = tuple(tiles[:3])
     else:
         tiles = s.autotune(3) if autotune else None
@@ -87,6 +89,7 @@ def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_s
     fuse = s.get_tile(hos.HOS_TILE_FUSE)
     skew = s.get_tile(hos.HOS_TILE_SKEW)
     diag = s.get_tile(hos.HOS_TILE_DIAG)
+    xblock = s.get_tile(hos.HOS_TILE_XBLOCK)
     s.use_graph(graph)
     s.step(warmup, dt)
     s.sync()
@@ -112,6 +115,7 @@ def run_form(hos, w, prec, members, steps, warmup, dt, u0, graph=True, profile_s
         "fuse": fuse == 2 and diag != 2,
         "skew": skew == 2 and groups > 1,
         "diag": diag == 2,
+        "xblock": xblock == 2 and diag == 2 and xpass == 2 and p.nlevs >= 1,
     }
     if profile_steps:
         # same workload, stream-ordered launches with a HIP event pair around
@@ -267,7 +271,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="arrival window of the Poisson serving workload (0: skip it)")
     ap.add_argument("--tiles", type=str, default=None,
-                    help="fixed launch shapes c2r,r2c,pair[,xpass[,groups[,plan[,fuse[,skew[,diag]]]]]] for the batched form instead of the autotuner "
+                    help="fixed launch shapes c2r,r2c,pair[,xpass[,groups[,plan[,fuse[,skew[,diag[,xblock]]]]]]] for the batched form instead of the autotuner "
                          "(profiler runs: no tuning trials in the kernel statistics)")
     ap.add_argument("--calib", action="store_true", help="only the plain copy kernel (counter calibration)")
     ap.add_argument("--traffic", nargs=4, metavar=("FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
@@ -382,6 +386,7 @@ def main():
             "rhs_fft_plan": {1: "built-in rule", 2: "radix <= 8", 3: "radix 4", 4: "largest radix"}.get(f32["plan"],
                                                                                                     f32["plan"]),
             "w_expansion": "diagonal passes" if f32["diag"] else "level passes",
+            "x_intermediates": "blocked: 4-row column-pair chunks" if f32["xblock"] else "row by row",
             "fused_level_passes": f32["fuse"],
             "offset_member_groups": f32["skew"],
             "form": "FP32 FFTs/products with Kahan order sums; FP64 state, RK combination, linear terms",
