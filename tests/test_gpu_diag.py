@@ -183,7 +183,7 @@ def test_diag_large_batch_matches_mode_off(hos):
 
 
 @pytest.mark.parametrize("Nx,Ny", [(512, 64), (1024, 32), (512, 512)])
-@pytest.mark.parametrize("nlevs", [1, 2, 4])
+@pytest.mark.parametrize("nlevs", [1, 2, 4, 7])
 @pytest.mark.parametrize("band", [1, 2])
 def test_xblock_same_bits(hos, Nx, Ny, nlevs, band):
     """Blocked x-intermediates (HOS_TILE_XBLOCK) change only where the values
