@@ -1699,7 +1699,6 @@ static_assert(XW == XKB, "a wave x-pass workgroup holds one block of rows");
 template <int LOGNX>
 constexpr int xblock_units() { return XKB * (1 << LOGNX) / 2 / NT; }
 
-
 // the workgroup's rows ky0 .. ky0+XKB-1 from the waves' registers (v, this
 // wave's row) to the blocked field dst: each wave puts its row in its LDS
 // row, then consecutive threads store consecutive 16-byte units of the
