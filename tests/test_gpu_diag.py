@@ -232,3 +232,27 @@ def test_xblock_rhs_vs_oracle(hos, oracle_mod, Nx, Ny):
     print(f"  rel L2 {err:.2e}", flush=True)
     assert err < TOL32, err
     s.close()
+
+
+def test_xblock_bench_shape_same_bits(hos):
+    """The benchmark's shape: eight 1024^2 members, M = 5, FP32, kp h = 1, two
+    member groups under captured graphs, masked states: three RK4 steps with
+    the blocked x-intermediates are the same bits as row by row."""
+    N, nlevs, nm = 1024, 4, 8
+    L = 64 * np.pi
+    print("xblock bench shape", flush=True)
+    states = [random_state(N, N, nlevs, amp=0.02, slope=3.0, seed=130 + i, Lx=L, Ly=L) for i in range(nm)]
+    out = {}
+    for xb in (1, 2):
+        s = solver(hos, N, N, nlevs, 1, nm=nm, Lx=L, Ly=L, depth=1.0)
+        s.set_tile(2, hos.HOS_TILE_XPASS)
+        s.set_tile(2, hos.HOS_TILE_GROUPS)
+        s.set_tile(xb, hos.HOS_TILE_XBLOCK)
+        s.use_graph(True)
+        for i, u in enumerate(states):
+            s.set_state(u, i)
+        s.step(3, 0.05)
+        out[xb] = [s.get_state(i) for i in range(nm)]
+        s.close()
+    for a, b in zip(out[1], out[2]):
+        assert np.array_equal(a, b)
