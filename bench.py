@@ -275,11 +275,15 @@ def main():
                          "(profiler runs: no tuning trials in the kernel statistics)")
     ap.add_argument("--calib", action="store_true", help="only the plain copy kernel (counter calibration)")
     ap.add_argument("--traffic", nargs=4, metavar=("FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
-                    help="summarise four rocprofv3 --pmc output folders into " + TRAFFIC_FILE)
+                    help="summarise four rocprofv3 --pmc output folders into --traffic-out")
+    ap.add_argument("--traffic-out", default=TRAFFIC_FILE, help="file the --traffic summary goes to")
+    ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32",
+                    help="arithmetic of the batched form (fp64: the FP64 form in its place, for profiling runs "
+                         "of the FP64 kernels with --members 1 --no-fp64)")
     args = ap.parse_args()
     if args.traffic:
         t = traffic_summary(*args.traffic)
-        with open(TRAFFIC_FILE, "w") as fh:
+        with open(args.traffic_out, "w") as fh:
             json.dump(t, fh, indent=1)
         print(json.dumps(t))
         return
@@ -308,9 +312,10 @@ def main():
     graph = not args.no_graph
 
     fixed = [int(v) for v in args.tiles.split(",")] if args.tiles else None
-    f32 = run_form(hos, w, hos.HOS_FP32, args.members, args.steps, args.warmup, dt, batch, graph=graph,
+    prec = hos.HOS_FP32 if args.precision == "fp32" else hos.HOS_FP64
+    f32 = run_form(hos, w, prec, args.members, args.steps, args.warmup, dt, batch, graph=graph,
                    profile_steps=args.profile_steps, tiles=fixed)
-    print(f"[bench] fp32 x{args.members}: {f32['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    print(f"[bench] {args.precision} x{args.members}: {f32['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     f1 = None
     if not args.no_single and args.members != 1:
         f1 = run_form(hos, w, hos.HOS_FP32, 1, args.steps, args.warmup, dt, u0, graph=graph)
@@ -334,7 +339,7 @@ def main():
         avg_bytes = k["bytes"] / max(k["launches"], 1)
         achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
         traffic = None
-        if args.members == 8 and os.path.exists(TRAFFIC_FILE):
+        if args.members == 8 and args.precision == "fp32" and os.path.exists(TRAFFIC_FILE):
             tk = json.load(open(TRAFFIC_FILE)).get("kernels", {}).get(name)
             traffic = round(tk["bytes_per_launch"]) if tk else None
         roof = {
@@ -371,7 +376,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": args.precision,
         "data": "synthetic seeded JONSWAP sea states (hos_ic_jonswap), resident in HBM",
         "config": {
             "workload": f"FP32 batched form: {args.members} realisations of the 1024x1024 grid over 32x32 peak "
