@@ -1052,6 +1052,21 @@ __device__ __forceinline__ void rhs_phase_b(cx<T> *__restrict__ out, int x0, int
 }
 
 
+// Deferred fields per LDS batch of the RHS pass: four in the FP32 form over
+// W fields at Ny >= 1024 (8 x 1024^2, M = 5, blocked x-intermediates: two
+// batches of four 101-103 us per launch against 108-110 with batches of 3, 3
+// and 2; the same VGPRs, 39 KB of LDS), three otherwise.  There a workgroup
+// holds one column pair whatever the y tile (TXP = 2), so the batch and the
+// radix plan of its row FFTs (fft_stage_bits: the same for 2 to 4 rows of 1024
+// or 2048 points) do not depend on the tile.  At Ny = 256, 512 a four-row
+// batch would take another plan than a three-row one, and tiles of 4 and 8
+// column pairs other plans again.
+template <class T, bool WF, int LOGNY>
+constexpr int rhs_bf()
+{
+    return WF && sizeof(T) == 4 && LOGNY >= 10 ? 4 : 3;
+}
+
 // the physical fields are read before phase A (with the first deferred tiles,
 // one memory latency less per tile) where they are few: FP32 form, at most
 // ten fields (M <= 5, or W^(a) fields at any M); their sums then wait in LDS
@@ -1094,7 +1109,7 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
     for (int q = 0; q < PER; ++q) wek[q] = F(1);
     RhsPre<F, C> pre[PER];
     // deferred fields in batches of BF, the last one of NFL
-    constexpr int BF = 3, NB = (ND + BF - 1) / BF, NFL = ND - (NB - 1) * BF;
+    constexpr int BF = rhs_bf<T, WF, LOGNY>(), NB = (ND + BF - 1) / BF, NFL = ND - (NB - 1) * BF;
     C2rBatch<T, LOGNY, NPR, NTT, BF> tb;   // batches 0 .. NB-2
     C2rBatch<T, LOGNY, NPR, NTT, NFL> tl;  // batch NB-1
     // the twiddles to LDS (TwLds): the FFTs run while the next batch loads
@@ -1148,7 +1163,9 @@ __device__ __forceinline__ void rhs_tile(const Phys<T> &ph, const HosIdx &ix, co
             tb.load(Im, fsI, dfr.slot + (b + 1) * BF, dfr.nzr + (b + 1) * BF, x0, nx, dfr.blk);
         else if (b + 1 < NB)
             tl.load(Im, fsI, dfr.slot + (b + 1) * BF, dfr.nzr + (b + 1) * BF, x0, nx, dfr.blk);
-        if (nf == 3)
+        if (nf == 4)
+            lds_fft<T, LOGNY, 4 * NPR, NTT, true, PLAN>(buf, twl);
+        else if (nf == 3)
             lds_fft<T, LOGNY, 3 * NPR, NTT, true, PLAN>(buf, twl);
         else if (nf == 2)
             lds_fft<T, LOGNY, 2 * NPR, NTT, true, PLAN>(buf, twl);
@@ -1225,8 +1242,9 @@ __global__ __launch_bounds__((rhs_nt<T, WF>())) void k_yr2c_rhs(Phys<T> ph, HosI
 {
     static_assert(TXP % 2 == 0, "x-rows in pairs");
     constexpr int NY = 1 << LOGNY;
-    // the r2c of TXP rows, or a batch of 3 deferred fields of TXP/2 rows each
-    __shared__ cx<T> buf[Lds<T, NY>::size(TXP > 3 * (TXP / 2) ? TXP : 3 * (TXP / 2))];
+    // the r2c of TXP rows, or a batch of rhs_bf deferred fields of TXP/2 rows each
+    constexpr int BROWS = rhs_bf<T, WF, LOGNY>() * (TXP / 2);  // rows of the largest deferred batch
+    __shared__ cx<T> buf[Lds<T, NY>::size(TXP > BROWS ? TXP : BROWS)];
     // the early sums (rhs_early): up to 5 vectors per task
     constexpr int NTASK = TXP * (NY / Vec<T>::N);
     __shared__ vec_t<T> pbuf[rhs_early<T, M, WF>() ? 5 * NTASK : 1];
