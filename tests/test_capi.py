@@ -87,3 +87,14 @@ def test_bytes_per_point_model(hos):
     assert hos.bytes_per_point_step(p) == 4 * (26 * 16 + 64)
     p = hos.params(nlevs=4, precision=hos.HOS_FP64)
     assert hos.bytes_per_point_step(p) == 4 * (26 * 32 + 64)
+
+
+def test_python_constants_match_header(hos):
+    """Every HOS_* integer constant the Python module defines has the value the
+    C header gives it (tile kinds, kernel kinds, operators, precisions)."""
+    txt = open(os.path.join(ROOT, "include", "hos_gpu.h")).read()
+    defs = {k: int(v) for k, v in re.findall(r"#define\s+(HOS_[A-Z0-9_]+)\s+\(?(-?\d+)\)?\s", txt)}
+    mine = {k: getattr(hos, k) for k in dir(hos) if k.startswith("HOS_") and isinstance(getattr(hos, k), int)}
+    shared = sorted(set(defs) & set(mine))
+    assert {"HOS_TILE_DIAG", "HOS_TILE_XBLOCK", "HOS_TILE_PLAN", "HOS_FP32"} <= set(shared), shared
+    assert {k: mine[k] for k in shared} == {k: defs[k] for k in shared}
