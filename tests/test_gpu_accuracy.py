@@ -70,3 +70,36 @@ def test_fp64_one_step_vs_oracle_1024(hos, oracle_mod):
     err = rel_l2(s.get_state(), ref)
     print(f"fp64 1024^2 one RK4 step vs oracle: {err:.2e}")
     assert err < 1e-13
+
+
+def test_fp32_batch_1000_steps_vs_fp64(hos):
+    """The benchmark's form: eight distinct sea states (seeds 1-8) in one FP32
+    context with one row per wave in the x passes, blocked x-intermediates and
+    two member groups under graphs.  After 1000 RK4 steps members 0 and 7 agree
+    with the plain FP64 path of the same states to <= 1e-5 (eta and phi_s)."""
+    kp = 1.0
+    L = 32 * 2 * math.pi / kp
+    dt = hos.peak_period(9.81, kp, 1.0) / 100
+    nm = 8
+    states = [hos.jonswap(1024, 1024, L, L, kp=kp, gamma=5.0, eps=0.12, depth=1.0, seed=1 + m, nlevs=4)
+              for m in range(nm)]
+    print("accuracy 1000 steps fp32 batch of 8", flush=True)
+    p = hos.params(Nx=1024, Ny=1024, nlevs=4, Lx=L, Ly=L, depth=1.0, precision=hos.HOS_FP32, compensated=1)
+    s = hos.Solver(p, nm)
+    for m, u in enumerate(states):
+        s.set_state(u, m)
+    s.set_tile(2, hos.HOS_TILE_XPASS)
+    s.set_tile(2, hos.HOS_TILE_GROUPS)
+    s.set_tile(2, hos.HOS_TILE_XBLOCK)
+    s.use_graph(True)
+    for _ in range(10):
+        s.step(100, dt)
+    got = {m: s.get_state(m) for m in (0, nm - 1)}
+    s.close()
+    for m in (0, nm - 1):
+        print(f"  fp64 member {m}", flush=True)
+        u64 = run(hos, hos.HOS_FP64, states[m], L, dt, 1000)[0]
+        e_eta, e_phi = rel_l2(got[m][0], u64[0]), rel_l2(got[m][1], u64[1])
+        print(f"  member {m}: eta rel L2 {e_eta:.3e}, phi {e_phi:.3e}", flush=True)
+        assert e_eta <= 1e-5 and e_phi <= 1e-5
+        assert e_eta <= 2e-6 and e_phi <= 2e-6  # regression guard (the single-member form: 2.5e-7)
